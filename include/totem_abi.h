@@ -1,0 +1,153 @@
+/*
+ * totem_abi.h -- C ABI of the MI355X multislice STEM engine.
+ *
+ * Plain C types only: host pointers are double/int arrays laid out exactly as
+ * the reference's MATLAB driver hands them to the mex layer (row-major after
+ * the transposes in cuda_stem.m:22-79); device pointers are void*.
+ *
+ * Status convention for the totem_* functions: 0 on success, a negative code
+ * on failure, with a message available from totem_last_error().
+ */
+#ifndef TOTEM_ABI_H
+#define TOTEM_ABI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TOTEM_OK 0
+#define TOTEM_ERR_ARG (-1)      /* invalid argument / shape / window */
+#define TOTEM_ERR_DEVICE (-2)   /* HIP runtime error */
+#define TOTEM_ERR_MEMORY (-3)   /* device allocation failed or workspace too small */
+#define TOTEM_ERR_UNSUPPORTED (-4) /* grid size without a compiled FFT plan, non-separable k grids */
+
+/* ------------------------------------------------------------------ drop-in
+ * Replaces the reference's
+ *   extern void myProjectfun(...)           waveCuda.h:10-21
+ * implemented four times:
+ *   cuda_source_lobato.cu:1018               (STEM, Lobato factors)
+ *   cuda_source_peng_and_corr.cu:1217        (STEM, Peng + correction)
+ *   cuda_source_lobato_potential.cu:747      (transmission only, Lobato)
+ *   cuda_source_peng_and_corr_potential.cu:952 (transmission only, Peng + correction)
+ * Each variant library (libwaveCudalobato.so, libwaveCudaPengAndCorr.so,
+ * libwaveCudalobatoP.so, libwaveCudaPengAndCorrP.so) exports this one symbol
+ * with C linkage, as the four mex products of mex_cuda.m:19-38 do.  The
+ * function returns nothing (like the reference); failures are reported through
+ * totem_last_error() and leave the outputs zeroed.
+ */
+void myProjectfun(double *my_real_w, double *my_imag_w, double *aper,
+                  double *atom_slice, double *absorp_n, double *atom_slice_i,
+                  double *absorp_n_i, double *proj_coff_mat, double *proj_coff_mat_i,
+                  double *kx, double *ky, double *s_2, double *my_real_const,
+                  double *my_img_const, double *gfsf, int Height, int Width, int Depth,
+                  int P_Height, int P_Width, int Slices, int step, int beginRow, int beginCol,
+                  int width_red, double sigma, int w_num, int aper_num, double parameter,
+                  double *AperTrue, int *mid_ceng, int ceng_len, double *aper2,
+                  double *series_n_corr, double *series_n_i_corr,
+                  double *ele_n_corr, double *ele_n_i_corr,
+                  double *corr_info_matrix, int cim_len,
+                  double *return_result, double *mid_ceng_mat, double *potentialx, double *potentialy);
+
+/* The four variants with a status return (same 43 parameters as above).
+ * STEM variants write return_result [aper_num][Depth] and mid_ceng_mat
+ * [ceng_len][aper_num][Depth]; potential variants write the transmission
+ * function t = exp(i sigma/1000 V) as potentialx/potentialy [Slices][P_Height][P_Width]. */
+#define TOTEM_MYPROJECTFUN_PARAMS \
+  double *my_real_w, double *my_imag_w, double *aper, double *atom_slice, double *absorp_n, \
+  double *atom_slice_i, double *absorp_n_i, double *proj_coff_mat, double *proj_coff_mat_i, \
+  double *kx, double *ky, double *s_2, double *my_real_const, double *my_img_const, double *gfsf, \
+  int Height, int Width, int Depth, int P_Height, int P_Width, int Slices, int step, int beginRow, \
+  int beginCol, int width_red, double sigma, int w_num, int aper_num, double parameter, \
+  double *AperTrue, int *mid_ceng, int ceng_len, double *aper2, double *series_n_corr, \
+  double *series_n_i_corr, double *ele_n_corr, double *ele_n_i_corr, double *corr_info_matrix, \
+  int cim_len, double *return_result, double *mid_ceng_mat, double *potentialx, double *potentialy
+
+int totem_project_lobato(TOTEM_MYPROJECTFUN_PARAMS);   /* cuda_source_lobato.cu:1018 */
+int totem_project_peng(TOTEM_MYPROJECTFUN_PARAMS);     /* cuda_source_peng_and_corr.cu:1217 */
+int totem_potential_lobato(TOTEM_MYPROJECTFUN_PARAMS); /* cuda_source_lobato_potential.cu:747 */
+int totem_potential_peng(TOTEM_MYPROJECTFUN_PARAMS);   /* cuda_source_peng_and_corr_potential.cu:952 */
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *totem_last_error(void);
+/* ABI version of this library. */
+int totem_abi_version(void);
+/* 1 if an n-point FFT plan is compiled in. */
+int totem_fft_size_supported(int n);
+
+/* ------------------------------------------------------------ potential build
+ * The potential part of myProjectfun (cal_p/initKernel_P, cuda_source_lobato.cu:585-956;
+ * p_corr, cuda_source_peng_and_corr.cu:652-783) with the transmission stack
+ * left on the device.  Host arrays in the myProjectfun layouts. */
+typedef struct totem_potential_desc {
+  int paraflag;                   /* 'l' Lobato (14/26-column rows), 'p' Peng, 'n' Peng + correction (13/25) */
+  int np;                         /* P_Height == P_Width */
+  int slices;
+  const double *atom_slice;       /* [S] rows of proj_coff_mat per slice */
+  const double *proj_coff_mat;    /* [sum][14 | 13] */
+  const double *absorp_n;         /* [sum][10] or NULL */
+  const double *atom_slice_i;     /* [S] or NULL */
+  const double *proj_coff_mat_i;  /* [sum_i][26 | 25] or NULL */
+  const double *absorp_n_i;       /* [sum_i][10] or NULL */
+  const double *series_n_corr;    /* [S] or NULL */
+  const double *ele_n_corr;       /* [sum][5] = x y B occ*w class(1-based) */
+  const double *series_n_i_corr;  /* [S] or NULL */
+  const double *ele_n_i_corr;     /* [sum][8] = x y B occ w1 class1 w2 class2 */
+  const double *corr_info_matrix; /* [cim_len][np][np] or NULL */
+  int cim_len;
+  const double *kx, *ky, *s_2;    /* [np][np] centred grids (kx varies along columns) */
+  const double *aper_true;        /* [np][np] centred, multiplies V(k) */
+  double sigma;                   /* 1/(kV Angstrom); the engine uses sigma/1000 */
+  double parameter;               /* PARAMETER */
+} totem_potential_desc;
+
+/* Builds t = exp(i sigma/1000 V) into t_dev (complex64 [S][np][np], device).
+ * If v_re/v_im are non-NULL the real-space potential V (FP32 values) is also
+ * copied to them (host, [S][np][np]). */
+int totem_build_transmission(const totem_potential_desc *desc, void *t_dev, double *v_re, double *v_im,
+                             void *stream);
+
+/* --------------------------------------------------------- device STEM job
+ * The propagation part of myProjectfun with every array already in HBM. */
+typedef struct totem_stem_job {
+  int n;                 /* probe window side (Height == Width) */
+  int np;                /* potential side */
+  int slices;
+  const void *t;         /* complex64 [S][np][np] */
+  const void *probes;    /* complex64 [M][n][n] */
+  int n_probes;          /* M (focal-spread members) */
+  const void *prop;      /* complex64 [n][n], zero frequency at [0][0] */
+  const double *aper;    /* [A][n][n] centred masks */
+  const double *aper2;   /* [A][n][n] ifftshifted masks, required when n_mid > 0 */
+  int n_aper;            /* A */
+  const int *mid;        /* host [L], 1-based slice numbers */
+  int n_mid;             /* L */
+  int depth;             /* D scan positions */
+  int width_red;         /* scan columns */
+  int step;              /* scan step in pixels */
+  int begin_row, begin_col; /* 0-based window origin of position 0 */
+  int wrap;              /* 1: periodic window (mod np); 0: window must lie inside the potential */
+  double *det;           /* device [(L+1)][A][M*D]: mid layers 0..L-1, exit face at L;
+                            column m*D + d (no focal-spread weighting applied) */
+} totem_stem_job;
+
+#define TOTEM_RUN_GRAPH 1     /* capture the slice loop of a batch in a hipGraph and replay it */
+
+/* Device bytes of workspace totem_stem_run needs for `batch` probes in flight. */
+size_t totem_stem_workspace_bytes(const totem_stem_job *job, int batch);
+/* Runs all M*D probes of the job, `batch` at a time (batch = 1 is the plain
+ * one-probe-per-launch form).  workspace may be NULL (the library allocates).
+ * Asynchronous on `stream` (NULL: the library's stream) unless it allocates. */
+int totem_stem_run(const totem_stem_job *job, int batch, int flags, void *workspace, size_t ws_bytes,
+                   void *stream);
+
+/* Batched in-place 2-D FFT of `batch` n x n complex64 matrices on the device
+ * (sign -1 forward, +1 inverse, unnormalised).  Test/diagnostic entry point. */
+int totem_fft2d(void *data, int n, int batch, int sign, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOTEM_ABI_H */

@@ -1,0 +1,267 @@
+"""GPU parity: the HIP engine through its C ABI against the FP64 oracle on identical inputs.
+
+Tolerances (stated by the engine's contract): potential slices <= 1e-6 relative L2,
+transmission <= 1e-6 relative L2, annular-detector images <= 1e-4 relative per pixel;
+batched / plain / graph-replayed runs are bit-identical.  FP64-oracle errors are
+printed for every case.
+"""
+import numpy as np
+import pytest
+
+from oracle import multislice as oms
+from oracle import potential as opot
+from totem_amd import optics, specimen, stem
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch
+
+
+def _stream():
+    return _torch().cuda.current_stream().cuda_stream
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+# ---------------------------------------------------------------------------- FFT
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 768, 1024, 1536, 2048])
+def test_fft2d_matches_numpy(engine, n):
+    torch = _torch()
+    print("case fft2d n=%d" % n, flush=True)
+    rng = np.random.default_rng(n)
+    batch = 2 if n <= 1024 else 1
+    x = (rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))).astype(np.complex64)
+    for sign in (-1, +1):
+        d = torch.tensor(x, device="cuda")
+        engine.fft2d(d.data_ptr(), n, batch, sign, stream=_stream())
+        torch.cuda.synchronize()
+        y = d.cpu().numpy().astype(np.complex128)
+        ref = np.fft.fft2(x.astype(np.complex128)) if sign < 0 else np.fft.ifft2(x.astype(np.complex128)) * n * n
+        err = max(_rel_l2(y[b], ref[b]) for b in range(batch))
+        print("  sign %+d rel L2 %.3e" % (sign, err))
+        assert err < 2e-6
+
+
+# ---------------------------------------------------------------------- potential
+
+def _displaced_perovskite(nx, ny, nz, seed, occ=True):
+    a = 3.905
+    x = specimen.perovskite(nx, ny, nz, a=a)
+    rng = np.random.default_rng(seed)
+    x[:, 1:3] += rng.normal(0, 0.07, (len(x), 2))
+    if occ:
+        x[::7, 7] = 0.8
+    return x
+
+
+def _potential_case(flag, np_side=128, nx=2, S=4, ions=False, absorption=False, seed=1):
+    a = 3.905
+    x = _displaced_perovskite(nx, nx, S // 2, seed)
+    if ions:
+        o = np.nonzero(x[:, 0] == 8)[0][::3]
+        x[o, 4] = -2
+        x[o, 5] = 0.5
+    dz = a / 2
+    cfg = stem.StemConfig(atoms=x, np_side=np_side, dx=nx * a / np_side, n=64, kv=300.0, alpha_mrad=24.0,
+                          slice_thickness=dz, n_slices=S, slice_origin=-dz / 4, paraflag=flag)
+    inp = stem.prepare(cfg)
+    absorp = None
+    if absorption:
+        absorp = np.tile(np.array([0.01, 2.0, 0.02, 5.0, 0.005, 20.0, 0.0, 1.0, 0.0, 1.0]), (len(inp.ele_n), 1))
+    return cfg, inp, absorp
+
+
+def _build_gpu(engine, inp, absorp=None):
+    torch = _torch()
+    c = inp.cfg
+    S, npx = c.n_slices, c.np_side
+    t = torch.empty((S, npx, npx), dtype=torch.complex64, device="cuda")
+    corr = c.paraflag == "n"
+    vre, vim = engine.build_transmission(
+        c.paraflag, npx, S, inp.series_n, inp.ele_n, inp.gx, inp.gy, inp.s2, inp.aper_true, inp.sigma,
+        inp.parameter, t.data_ptr(), absorp_n=absorp,
+        atom_slice_i=inp.series_n_i if len(inp.series_n_i) else None,
+        proj_coff_mat_i=inp.ele_n_i if len(inp.ele_n_i) else None,
+        series_n_corr=inp.series_n_corr if corr else None, ele_n_corr=inp.ele_n_corr if corr else None,
+        corr_info_matrix=inp.corr_matrix if corr else None, want_v=True, stream=_stream())
+    torch.cuda.synchronize()
+    return vre + 1j * vim, t.cpu().numpy().astype(np.complex128)
+
+
+def _oracle_potential(inp, absorp=None):
+    c = inp.cfg
+    kw = {}
+    if c.paraflag == "n":
+        kw = dict(ele_n_corr=inp.ele_n_corr, corr_matrix=inp.corr_matrix, series_n_corr=inp.series_n_corr)
+    return opot.potential_slices(c.np_side, inp.ele_n, absorp, inp.ele_n_i, None, inp.series_n, inp.series_n_i,
+                                 inp.s2, inp.gx, inp.gy, inp.sigma, inp.parameter, inp.aper_true, c.paraflag, **kw)
+
+
+@pytest.mark.parametrize("flag,ions,absorption", [("l", False, False), ("p", False, False), ("n", False, False),
+                                                  ("p", True, True), ("l", False, True)])
+def test_potential_matches_oracle(engine, flag, ions, absorption):
+    print("case potential flag=%s ions=%s absorption=%s" % (flag, ions, absorption), flush=True)
+    cfg, inp, absorp = _potential_case(flag, ions=ions, absorption=absorption)
+    if ions:
+        assert len(inp.ele_n_i) > 0
+    V, t = _build_gpu(engine, inp, absorp)
+    Vo, to = _oracle_potential(inp, absorp)
+    for s in range(cfg.n_slices):
+        ev, et = _rel_l2(V[s], Vo[s]), _rel_l2(t[s], to[s])
+        print("  slice %d: V rel L2 %.3e  t rel L2 %.3e" % (s, ev, et))
+        assert ev <= 1e-6
+        assert et <= 1e-6
+
+
+def test_potential_only_entry_point_matches_oracle(engine):
+    print("case waveCudaPengAndCorrP", flush=True)
+    cfg, inp, _ = _potential_case("n")
+    cfg.width_red = cfg.hight_red = 1
+    _, _, px, py = stem.run(inp, potential_only=True)
+    _, to = _oracle_potential(inp)
+    S, npx = cfg.n_slices, cfg.np_side
+    t = (px + 1j * py).T.reshape(S, npx, npx)
+    err = _rel_l2(t, to)
+    print("  t rel L2 %.3e" % err)
+    assert err <= 1e-6
+
+
+# --------------------------------------------------------------------------- STEM
+
+def _stem_case(flag, M=1, mid=(), n=128, np_side=256, S=6, cells=4, scan=(3, 2), step=5, begin=(20, 30), seed=4):
+    a = 3.905
+    x = _displaced_perovskite(cells, cells, S // 2, seed)
+    dz = a / 2
+    ab = optics.Aberrations(C1=-5.0, Cs_um=1.0) if M > 1 else None
+    cfg = stem.StemConfig(atoms=x, np_side=np_side, dx=cells * a / np_side, n=n, kv=300.0 if flag != "l" else 200.0,
+                          alpha_mrad=24.0, slice_thickness=dz, n_slices=S, slice_origin=-dz / 4, paraflag=flag,
+                          aberrations=ab, focal_members=M, focal_rms_nm=3.0,
+                          detectors_mrad=((60.0, 200.0), (0.0, 10.0), (20.0, 40.0)), width_red=scan[0],
+                          hight_red=scan[1], step=step, begin_row=begin[0], begin_col=begin[1], mid_slices=mid)
+    return cfg, stem.prepare(cfg)
+
+
+def _oracle_stem(inp, t=None):
+    c = inp.cfg
+    if t is None:
+        _, t = _oracle_potential(inp)
+    return oms.stem_cpu(t, inp.probe, inp.gfsf, inp.psf, c.begin_row, c.begin_col, c.width_red, c.hight_red,
+                        c.step, inp.aper, tuple(sorted(c.mid_slices)))
+
+
+def _check_images(res, ref, what):
+    rel = np.abs(res - ref) / np.abs(ref)
+    print("  %s: max rel per pixel %.3e (FP64-oracle error), min value %.3e" % (what, rel.max(), np.abs(ref).min()))
+    assert rel.max() <= 1e-4
+
+
+@pytest.mark.parametrize("flag,M,mid", [("l", 1, ()), ("n", 3, (2, 5)), ("p", 1, (3,))])
+def test_stem_myprojectfun_matches_oracle(engine, flag, M, mid):
+    print("case stem flag=%s M=%d mid=%s" % (flag, M, mid), flush=True)
+    cfg, inp = _stem_case(flag, M=M, mid=mid)
+    myresul, mid_mat, px, py = stem.run(inp)
+    res_o, mid_o = _oracle_stem(inp)
+    assert myresul.shape == res_o.shape and mid_mat.shape == mid_o.shape
+    _check_images(myresul, res_o, "exit-face detectors")
+    if mid:
+        _check_images(mid_mat, mid_o, "mid-slice detectors")
+    assert np.all(px == 0) and np.all(py == 0)
+
+
+def test_stem_window_outside_potential_is_an_error(engine):
+    cfg, inp = _stem_case("l", scan=(3, 2), step=60, begin=(20, 30))
+    with pytest.raises(engine.TotemError, match="window leaves the potential"):
+        stem.run(inp)
+
+
+def _device_job(engine, inp, t_dev, wrap=False, begin=None):
+    torch = _torch()
+    c = inp.cfg
+    probes = torch.tensor(inp.probe.astype(np.complex64), device="cuda")
+    prop = torch.tensor(inp.psf.astype(np.complex64), device="cuda")
+    aper = torch.tensor(inp.aper, device="cuda")
+    aper2 = torch.tensor(inp.aper2, device="cuda")
+    L, A, M = len(c.mid_slices), inp.aper.shape[0], len(inp.gfsf)
+    det = torch.zeros(((L + 1), A, M * c.depth), dtype=torch.float64, device="cuda")
+    br, bc = (c.begin_row - 1, c.begin_col - 1) if begin is None else begin
+    job = engine.stem_job(c.n, c.np_side, c.n_slices, t_dev.data_ptr(), probes.data_ptr(), M, prop.data_ptr(),
+                          aper.data_ptr(), A, c.depth, c.width_red, c.step, br, bc, det.data_ptr(),
+                          aper2_ptr=aper2.data_ptr(), mid=sorted(c.mid_slices), wrap=wrap)
+    job._keep = (probes, prop, aper, aper2)
+    return job, det
+
+
+def test_batched_plain_and_graph_runs_are_bit_identical(engine):
+    torch = _torch()
+    print("case batch/plain/graph", flush=True)
+    cfg, inp = _stem_case("l", M=1, mid=(3,), scan=(4, 3), step=4)
+    _, to = _oracle_potential(inp)
+    t_dev = torch.tensor(to.astype(np.complex64), device="cuda")
+    outs = []
+    for batch, graph in ((cfg.depth, False), (1, False), (5, False), (cfg.depth, True), (5, True)):
+        job, det = _device_job(engine, inp, t_dev)
+        engine.stem_run(job, batch, graph=graph, stream=_stream())
+        torch.cuda.synchronize()
+        outs.append(det.cpu().numpy())
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+    # and the device path agrees with the oracle fed the same (FP32-rounded) transmission
+    res_o, mid_o = _oracle_stem(inp, t=to.astype(np.complex64).astype(np.complex128))
+    _check_images(outs[0][1].T, res_o, "device API exit face")
+
+
+def test_periodic_windows_equal_tiled_potential(engine):
+    torch = _torch()
+    print("case periodic windows", flush=True)
+    cfg, inp = _stem_case("l", n=128, np_side=128, cells=2, S=4, scan=(3, 3), step=40, begin=(1, 1))
+    _, to = _oracle_potential(inp)
+    t_dev = torch.tensor(to.astype(np.complex64), device="cuda")
+    job, det = _device_job(engine, inp, t_dev, wrap=True, begin=(100, 90))
+    engine.stem_run(job, cfg.depth, stream=_stream())
+    torch.cuda.synchronize()
+    tiled = np.tile(to.astype(np.complex64).astype(np.complex128), (1, 2, 2))
+    ref = np.zeros((cfg.depth, inp.aper.shape[0]))
+    for d in range(cfg.depth):
+        r0 = (100 + (d // cfg.width_red) * cfg.step) % 128
+        c0 = (90 + (d % cfg.width_red) * cfg.step) % 128
+        r, _ = oms.stem_position(tiled, inp.probe, inp.gfsf, inp.psf, np.fft.ifftshift(inp.aper, axes=(-2, -1)),
+                                 r0, c0)
+        ref[d] = r
+    _check_images(det.cpu().numpy()[0].T, ref, "periodic windows")
+
+
+def test_full_size_vacuum_closed_form(engine):
+    """1024^2, 80 slices, periodic windows: for t = 1 the exit-face detectors equal
+    sum_k |FFT(probe)|^2 over (band limit x mask), and a full-band BF collects N^2."""
+    torch = _torch()
+    print("case vacuum 1024^2 x 80 slices", flush=True)
+    n, S, dx, kv = 1024, 80, 78.1 / 1024, 300.0
+    probe, gfsf = optics.probe_stack(n, dx, kv, 24.0, optics.Aberrations(C1=-5.0, Cs_um=1.0))
+    psf = optics.propagator(n, dx, kv, 3.905 / 2)
+    aper = optics.detector_masks(n, dx, kv, [(70.0, 200.0), (0.0, 10.0), (0.0, 500.0)])
+    t = torch.ones((S, n, n), dtype=torch.complex64, device="cuda")
+    cfg = stem.StemConfig(atoms=np.zeros((0, 8)), np_side=n, dx=dx, n=n, kv=kv, alpha_mrad=24, slice_thickness=1.95,
+                          n_slices=S, width_red=2, hight_red=2, step=3)
+    none = {k: None for k in ("ele_n", "ele_n_i", "series_n", "series_n_i", "ele_n_corr", "series_n_corr",
+                              "corr_info", "corr_matrix", "gx", "gy", "s2", "aper_true", "parameter", "sigma")}
+    inp = stem.StemInputs(cfg=cfg, probe=probe, gfsf=gfsf, psf=psf, aper=aper,
+                          aper2=np.fft.ifftshift(aper, axes=(-2, -1)), **none)
+    job, det = _device_job(engine, inp, t, wrap=True, begin=(0, 0))
+    engine.stem_run(job, 4, stream=_stream())
+    torch.cuda.synchronize()
+    out = det.cpu().numpy()[0]
+    P = np.fft.fftshift(np.fft.fft2(probe[0]))
+    band = np.fft.fftshift(np.abs(psf) > 0)
+    closed = np.array([np.sum(np.abs(P) ** 2 * band * aper[a]) for a in range(3)])
+    for d in range(4):
+        rel = np.abs(out[:, d] - closed) / closed
+        print("  position %d rel err %s" % (d, rel))
+        assert rel.max() < 1e-4
+    assert out[2, 0] / n ** 2 == pytest.approx(1.0, abs=1e-4)

@@ -1,0 +1,196 @@
+"""CPU tests of the host side: tables, optics set-up, specimen preparation, the C-ABI
+library's exports and the mex gateway's argument mapping.  No GPU calls."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from totem_amd import optics, specimen, stem, tables
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# --------------------------------------------------------------------------- tables
+
+def test_table_shapes_and_known_values():
+    assert tables.lobato_table().shape == (103, 10)
+    assert tables.peng_table().shape == (98, 10)
+    # f_Sr(0): Lobato 2*sum(a) and Peng sum(a), both ~13.11 Angstrom
+    assert 2 * tables.lobato_table()[37, :5].sum() == pytest.approx(13.10990, abs=1e-5)
+    assert tables.peng_table()[37, :5].sum() == pytest.approx(13.1088, abs=1e-4)
+
+
+@pytest.mark.parametrize("Z", [6, 8, 14, 22, 38, 79])
+def test_lobato_and_peng_tables_agree(Z):
+    # two independent parameterisations of the same scattering factor (transcription check)
+    s = np.linspace(0, 2.0, 41)
+    fl = tables.lobato_f(Z, 4 * s * s)
+    fp = tables.peng_f(Z, s * s)
+    assert np.max(np.abs(fl - fp) / fl.max()) < 0.05
+
+
+# --------------------------------------------------------------------------- optics
+
+def test_wavelength_and_sigma_closed_forms():
+    # Kirkland, Advanced Computing in EM (sigma vs voltage), as ToTEM_submit.m:1490-1493 cross-checks
+    assert optics.wavelength_A(200) == pytest.approx(0.025078, abs=2e-6)
+    assert optics.wavelength_A(300) == pytest.approx(0.019687, abs=2e-6)
+    assert optics.interaction_sigma(200) == pytest.approx(0.728869, abs=2e-6)
+    assert optics.interaction_sigma(300) == pytest.approx(0.652640, abs=2e-6)
+
+
+def test_propagator_band_limit_and_unit_modulus():
+    n, dx = 64, 0.1
+    h = optics.propagator(n, dx, 200.0, 2.0)
+    f = np.fft.ifftshift(optics.centred_freq(n, dx))
+    kx, ky = np.meshgrid(f, f)
+    inside = kx ** 2 + ky ** 2 <= 1 / (9 * dx * dx)
+    np.testing.assert_allclose(np.abs(h[inside]), 1.0, atol=1e-14)
+    assert np.all(h[~inside] == 0)
+    # Fresnel composition: H(dz1) H(dz2) = H(dz1 + dz2) inside the band
+    h2 = optics.propagator(n, dx, 200.0, 3.0)
+    h5 = optics.propagator(n, dx, 200.0, 5.0)
+    np.testing.assert_allclose((h * h2)[inside], h5[inside], atol=1e-12)
+
+
+def test_detector_masks_and_probe_normalisation():
+    n, dx = 128, 0.08
+    aper = optics.detector_masks(n, dx, 200.0, [(60, 200), (0, 10)])
+    lam = optics.wavelength_A(200)
+    f = optics.centred_freq(n, dx)
+    kx, ky = np.meshgrid(f, f)
+    k = np.sqrt(kx ** 2 + ky ** 2) * lam * 1e3
+    assert aper[0][n // 2, n // 2] == 0 and aper[1][n // 2, n // 2] == 1
+    assert np.all(aper[0][(k > 61) & (k < 199)] == 1)
+    p, g = optics.probe_stack(n, dx, 200.0, 20.0, optics.Aberrations(C1=-5, Cs_um=1.0), mtotal=3, focal_rms_nm=4)
+    np.testing.assert_allclose(np.sum(np.abs(p) ** 2, axis=(1, 2)), 1.0, rtol=1e-12)
+    assert len(g) == 3 and g[0] == pytest.approx(g[2]) and g[1] > g[0]
+    # the probe's spectrum is confined to the aperture
+    P = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(p[1])))
+    out = k > 20.0 + 1e-9
+    assert np.max(np.abs(P[out])) < 1e-12 * np.max(np.abs(P))
+
+
+# ------------------------------------------------------------------------- specimen
+
+def test_perovskite_counts_and_slicing():
+    a = 3.905
+    x = specimen.perovskite(5, 5, 10, a=a)
+    assert x.shape == (1250, 8)
+    assert sorted(np.unique(x[:, 0])) == [8, 22, 38]
+    cfg = stem.perovskite_c1()
+    inp = stem.prepare(cfg)
+    assert inp.ele_n.shape == (1250, 14)
+    np.testing.assert_array_equal(inp.series_n, [50, 75] * 10)
+    # columns: x y occ a1 b1 ... a5 b5 B (Lobato); coefficients are the table's
+    sr = inp.ele_n[inp.ele_n[:, 3] == tables.lobato_table()[37, 0]]
+    assert len(sr) == 250
+    np.testing.assert_array_equal(sr[0, 3:13], tables.interleaved(tables.lobato_table()[37]))
+
+
+def test_multiceng_weights_sum_to_one():
+    # an atom split over neighbouring slices keeps its total weight (erf contributions)
+    a = 3.905
+    x = specimen.perovskite(1, 1, 3, a=a)
+    x[:, 6] = 0.5
+    all_n, all_n_i = specimen.atom_parameters(x, "p")
+    st = np.arange(6) * a / 2 - 0.3
+    ele_n, _, sn, _ = specimen.slice_atoms(all_n, all_n_i, a / 2, st, 1, "p", 0.0)
+    assert len(ele_n) > len(all_n)
+    per_atom = {}
+    for r in ele_n:
+        key = (round(r[0], 9), round(r[1], 9), round(r[3] / tables.peng_table()[int(0) + 7, 0], 3))
+        per_atom.setdefault((round(r[0], 9), round(r[1], 9)), []).append(r[3])
+    # sum of a1 weights over slices equals the table value for every column of atoms
+    tot = sum(ele_n[:, 3])
+    ref = sum(r[6] for r in all_n)
+    assert tot == pytest.approx(ref, rel=1e-9)
+
+
+def test_corr_slicing_classes():
+    x = specimen.perovskite(2, 2, 2)
+    cn = specimen.corr_parameters(x)
+    st = np.arange(4) * 3.905 / 2 - 0.4
+    ec, ci, sc = specimen.slice_corr(cn, 3.905 / 2, st, 0)
+    assert ci.shape == (3, 20) and ec.shape[1] == 5
+    assert set(np.unique(ec[:, 4])) == {1.0, 2.0, 3.0}
+    assert sc.sum() == len(x)
+    # sorted unique rows: MATLAB unique(...,'rows') orders lexicographically
+    assert np.all(np.diff(ci[:, 0]) > 0)
+
+
+def test_frozen_phonon_statistics_and_slab():
+    rng = np.random.default_rng(11)
+    x = specimen.perovskite(10, 10, 4)
+    all_n, _ = specimen.atom_parameters(x, "p")
+    B = 8 * np.pi ** 2 * 0.08 ** 2
+    all_n[:, 4] = B
+    st = np.arange(8) * 3.905 / 2 - 0.5
+    out = specimen.frozen_phonon(all_n, 3.905 / 2, st, (), rng)
+    d = out[:, 1:3] - all_n[:, 1:3]
+    assert np.std(d) == pytest.approx(0.08, rel=0.05)
+    assert np.all(out[:, 3] >= st[0]) and np.all(out[:, 3] < st[-1] + 3.905 / 2)
+
+
+def test_mex_args_layout():
+    cfg = stem.perovskite_c1(n=64, np_side=256, scan=2, step=3)
+    cfg.mid_slices = (5,)
+    inp = stem.prepare(cfg)
+    args = stem.mex_args(inp)
+    assert len(args) == 41
+    assert args[0].shape == (1, 64, 64) and args[2].shape == (1, 64, 64)
+    assert args[9].shape == (256, 256) and np.all(args[9][0] == args[9][5])  # kx varies along columns
+    assert args[32].dtype == np.int32 and list(args[32]) == [5]
+
+
+# -------------------------------------------------------------------------- C ABI
+
+def _header_functions():
+    src = open(os.path.join(ROOT, "include", "totem_abi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set(re.findall(r"^\s*(?:const\s+)?(?:void|int|size_t|char)\s*\*?\s*(\w+)\s*\(", src, flags=re.M))
+    return names
+
+
+def _exports(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    return {l.split()[-1] for l in out.splitlines() if " T " in l}
+
+
+def test_library_exports_every_declared_symbol():
+    from totem_amd import engine
+    names = _header_functions()
+    assert {"myProjectfun", "totem_project_lobato", "totem_stem_run", "totem_build_transmission"} <= names
+    core = _exports(engine.CORE)
+    assert names - {"myProjectfun"} <= core, names - core
+    for lib in engine.VARIANT_LIBS.values():
+        assert "myProjectfun" in _exports(os.path.join(engine.LIB_DIR, lib))
+    assert set(engine.EXPORTS) == names
+
+
+def test_library_loads_without_gpu():
+    from totem_amd import engine
+    L = ctypes.CDLL(engine.CORE)
+    L.totem_abi_version.restype = ctypes.c_int
+    assert L.totem_abi_version() == 1
+    for n in (64, 128, 256, 512, 768, 1024, 1536, 2048):
+        assert L.totem_fft_size_supported(n) == 1
+    assert L.totem_fft_size_supported(1000) == 0
+
+
+def test_mex_gateway_argument_mapping(tmp_path):
+    exe = tmp_path / "drive"
+    subprocess.run(["g++", "-O0", "-I" + os.path.join(ROOT, "tests", "mexstub"), "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "mexstub", "drive.cpp"),
+                    os.path.join(ROOT, "totem_amd", "mex", "waveCuda.cpp"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    assert "H=64 W=64 D=12 PH=256 PW=256 S=9 step=3 br=10 bc=20 wr=4 wn=3 an=2 cl=1 cim=0" in out
+    assert "absorp_null=1 absorp_i_null=0 corr_null=1 w0=0.25 aper2=0.5 mid=5" in out
+    assert "out0=12x2:42 out1=24x1 out2=65536x9:7" in out
+    assert "error=\n" in out
+    assert "nrhs40=Invalid number of input arguments" in out
+    out = subprocess.run([str(exe), "fail"], capture_output=True, text=True, check=True).stdout
+    assert "error=engine failure" in out

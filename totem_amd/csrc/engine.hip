@@ -1,0 +1,801 @@
+// Host orchestration and C ABI of the multislice STEM engine.
+//
+// myProjectfun semantics (reference cuda_source_lobato.cu:1018-1562 and
+// cuda_source_peng_and_corr.cu:1217-1786):
+//   1. transmission stack t[s] = exp(i sigma/1000 V_s) from the per-slice atom rows
+//   2. for every focal member m and scan position d: multislice through all
+//      slices, annular detectors at the exit face and at the listed mid slices
+//   3. return_result[a][d] = sum_m gfsf[m] I_m[a][d], mid_ceng_mat[l][a][d] likewise
+//
+// Design differences from the reference (results agree within FP32 rounding):
+//   * all probes of a batch advance one slice per pair of launches (rows pass,
+//     columns pass) instead of ~6 synchronised launches + 2 FFT plan builds;
+//   * the exit-face detector reads Psi*H inside the last columns pass
+//     (FFT(IFFT(Psi*H)) == Psi*H), so the last inverse and forward FFTs are skipped;
+//   * detector sums are per-workgroup FP64 partials reduced in a fixed order
+//     (deterministic), not a log2(N^2)-launch tree;
+//   * the potential is accumulated per pixel tile over atoms grouped by
+//     form-factor class (no cross-atom read-modify-write race).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/totem_abi.h"
+#include "kernels.h"
+#include "launch.h"
+
+namespace totem {
+
+// ----------------------------------------------------------------------- errors
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+static thread_local std::string g_last_error;
+
+#define HIP_OK(expr)                                                                                     \
+  do {                                                                                                   \
+    hipError_t e_ = (expr);                                                                              \
+    if (e_ != hipSuccess)                                                                                \
+      throw Error(e_ == hipErrorOutOfMemory ? TOTEM_ERR_MEMORY : TOTEM_ERR_DEVICE,                       \
+                  std::string(#expr) + " failed: " + hipGetErrorString(e_));                            \
+  } while (0)
+
+static void require(bool ok, const std::string& msg, int code = TOTEM_ERR_ARG) {
+  if (!ok) throw Error(code, msg);
+}
+
+template <class F> static int guarded(F&& f) {
+  try {
+    f();
+    g_last_error.clear();
+    return TOTEM_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return TOTEM_ERR_DEVICE;
+  }
+}
+
+// ----------------------------------------------------------------- device memory
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b) { alloc(b); }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) return;
+    HIP_OK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T> static void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+  b.alloc(std::max<size_t>(v.size(), 1) * sizeof(T));
+  if (!v.empty()) HIP_OK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+// Process-wide resident context: twiddle tables per (device, n) and one stream per device.
+struct Context {
+  std::mutex mu;
+  std::map<std::pair<int, int>, void*> tw;
+  std::map<int, hipStream_t> streams;
+};
+static Context& ctx() {
+  static Context* c = new Context();  // never destroyed: buffers live until process exit
+  return *c;
+}
+
+const float2* tw_table(int n) {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto key = std::make_pair(dev, n);
+  auto it = c.tw.find(key);
+  if (it != c.tw.end()) return static_cast<const float2*>(it->second);
+  std::vector<float2> h(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)n;
+    h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  void* p = nullptr;
+  HIP_OK(hipMalloc(&p, n * sizeof(float2)));
+  HIP_OK(hipMemcpy(p, h.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+  c.tw[key] = p;
+  return static_cast<const float2*>(p);
+}
+
+static hipStream_t lib_stream() {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  Context& c = ctx();
+  std::lock_guard<std::mutex> lk(c.mu);
+  auto it = c.streams.find(dev);
+  if (it != c.streams.end()) return it->second;
+  hipStream_t s;
+  HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  c.streams[dev] = s;
+  return s;
+}
+
+// The reference picks the GPU with the most free memory and makes it current
+// (cuda_source_lobato.cu:1063-1081).  TOTEM_DEVICE overrides the choice.
+static void select_device() {
+  int count = 0;
+  HIP_OK(hipGetDeviceCount(&count));
+  require(count > 0, "no HIP device", TOTEM_ERR_DEVICE);
+  if (const char* env = std::getenv("TOTEM_DEVICE")) {
+    HIP_OK(hipSetDevice(std::atoi(env)));
+    return;
+  }
+  int best = 0;
+  size_t best_free = 0;
+  if (count > 1) {
+    for (int i = 0; i < count; ++i) {
+      HIP_OK(hipSetDevice(i));
+      size_t fr = 0, tot = 0;
+      HIP_OK(hipMemGetInfo(&fr, &tot));
+      if (fr > best_free) {
+        best_free = fr;
+        best = i;
+      }
+    }
+  }
+  HIP_OK(hipSetDevice(best));
+}
+
+// -------------------------------------------------------------- potential build
+
+struct ClassKey {
+  std::vector<double> v;
+  bool operator<(const ClassKey& o) const { return v < o.v; }
+};
+
+struct Term {
+  double x, y, w;
+  int cls;
+};
+
+static PotClass make_class(const std::vector<double>& key) {
+  PotClass k;
+  std::memset(&k, 0, sizeof(k));
+  k.kind = (int)key[0];
+  k.has_abs = (int)key[1];
+  k.i1 = (int)key[2];
+  k.i2 = (int)key[3];
+  k.w1 = key[4];
+  k.w2 = key[5];
+  k.B = key[6];
+  for (int q = 0; q < 10; ++q) {
+    k.p1[q] = key[7 + q];
+    k.p2[q] = key[17 + q];
+    k.ab[q] = key[27 + q];
+  }
+  return k;
+}
+
+static std::vector<double> blank_key(int kind) {
+  std::vector<double> k(37, 0.0);
+  k[0] = kind;
+  return k;
+}
+
+static void counts_to_offsets(const double* counts, int S, std::vector<long>& off, const char* what) {
+  off.assign(S + 1, 0);
+  for (int s = 0; s < S; ++s) {
+    const double c = counts ? counts[s] : 0.0;
+    require(c >= 0 && c == std::floor(c), std::string(what) + ": per-slice counts must be non-negative integers");
+    off[s + 1] = off[s] + (long)c;
+  }
+}
+
+struct PotentialPlan {
+  int np = 0, S = 0;
+  std::vector<double> kx1, ky1;
+  std::vector<std::vector<Term>> slices;
+  std::vector<PotClass> classes;
+};
+
+static PotentialPlan plan_potential(const totem_potential_desc& d) {
+  PotentialPlan P;
+  const int np = d.np, S = d.slices;
+  require(d.paraflag == 'l' || d.paraflag == 'p' || d.paraflag == 'n', "paraflag must be 'l', 'p' or 'n'");
+  require(np > 0 && np % 2 == 0, "P_Height/P_Width must be positive and even");
+  require(S > 0, "Slices must be positive");
+  require(d.kx && d.ky && d.s_2 && d.aper_true, "kx, ky, s_2 and AperTrue are required");
+  require(size_supported(np), "potential grid " + std::to_string(np) + " has no compiled FFT plan",
+          TOTEM_ERR_UNSUPPORTED);
+  P.np = np;
+  P.S = S;
+  // The reference grids come from meshgrid: kx depends on the column, ky on the row.
+  P.kx1.assign(d.kx, d.kx + np);
+  P.ky1.resize(np);
+  for (int r = 0; r < np; ++r) P.ky1[r] = d.ky[(size_t)r * np];
+  for (int r = 0; r < np; ++r)
+    for (int c = 0; c < np; ++c) {
+      const size_t i = (size_t)r * np + c;
+      require(d.kx[i] == P.kx1[c] && d.ky[i] == P.ky1[r],
+              "kx/ky are not meshgrid-separable (kx must vary along columns, ky along rows)",
+              TOTEM_ERR_UNSUPPORTED);
+    }
+  const bool lob = d.paraflag == 'l';
+  const int ncol = lob ? 14 : 13, ncol_i = lob ? 26 : 25;
+  std::map<ClassKey, int> cmap;
+  auto class_of = [&](std::vector<double>&& key) {
+    ClassKey k{std::move(key)};
+    auto it = cmap.find(k);
+    if (it != cmap.end()) return it->second;
+    const int id = (int)P.classes.size();
+    P.classes.push_back(make_class(k.v));
+    cmap.emplace(std::move(k), id);
+    return id;
+  };
+  P.slices.assign(S, {});
+  std::vector<long> off;
+  // atoms with a single (atomic or ionic) form factor
+  counts_to_offsets(d.atom_slice, S, off, "atom_slice");
+  if (off[S] > 0) require(d.proj_coff_mat != nullptr, "proj_coff_mat missing");
+  for (int s = 0; s < S; ++s)
+    for (long a = off[s]; a < off[s + 1]; ++a) {
+      const double* r = d.proj_coff_mat + a * ncol;
+      auto key = blank_key(lob ? kLobato : kPeng);
+      for (int q = 0; q < 10; ++q) key[7 + q] = r[3 + q];
+      if (lob) key[6] = r[13];
+      if (d.absorp_n) {
+        key[1] = 1;
+        for (int q = 0; q < 10; ++q) key[27 + q] = d.absorp_n[a * 10 + q];
+      }
+      P.slices[s].push_back({r[0], r[1], r[2], class_of(std::move(key))});
+    }
+  // atoms with mixed atomic + ionic form factors
+  counts_to_offsets(d.atom_slice_i, S, off, "atom_slice_i");
+  if (off[S] > 0) require(d.proj_coff_mat_i != nullptr, "proj_coff_mat_i missing");
+  for (int s = 0; s < S; ++s)
+    for (long a = off[s]; a < off[s + 1]; ++a) {
+      const double* r = d.proj_coff_mat_i + a * ncol_i;
+      auto key = blank_key(lob ? kLobatoIon : kPengIon);
+      key[4] = r[3];
+      for (int q = 0; q < 10; ++q) key[7 + q] = r[4 + q];
+      key[5] = r[14];
+      for (int q = 0; q < 10; ++q) key[17 + q] = r[15 + q];
+      if (lob) key[6] = r[25];
+      if (d.absorp_n_i) {
+        key[1] = 1;
+        for (int q = 0; q < 10; ++q) key[27 + q] = d.absorp_n_i[a * 10 + q];
+      }
+      P.slices[s].push_back({r[0], r[1], r[2], class_of(std::move(key))});
+    }
+  // correction terms (Peng "extended"), p_corr: cuda_source_peng_and_corr.cu:652-783
+  if (d.cim_len != 0) {
+    require(d.cim_len > 0 && d.corr_info_matrix != nullptr, "corr_info_matrix missing");
+    auto check_cls = [&](double c) {
+      require(c >= 1 && c <= d.cim_len && c == std::floor(c), "correction class index out of range");
+      return (int)c - 1;
+    };
+    counts_to_offsets(d.series_n_corr, S, off, "series_n_corr");
+    if (off[S] > 0) require(d.ele_n_corr != nullptr, "ele_n_corr missing");
+    for (int s = 0; s < S; ++s)
+      for (long a = off[s]; a < off[s + 1]; ++a) {
+        const double* r = d.ele_n_corr + a * 5;
+        auto key = blank_key(kCorr);
+        key[2] = check_cls(r[4]);
+        key[6] = r[2];
+        P.slices[s].push_back({r[0], r[1], r[3], class_of(std::move(key))});
+      }
+    counts_to_offsets(d.series_n_i_corr, S, off, "series_n_i_corr");
+    if (off[S] > 0) require(d.ele_n_i_corr != nullptr, "ele_n_i_corr missing");
+    for (int s = 0; s < S; ++s)
+      for (long a = off[s]; a < off[s + 1]; ++a) {
+        const double* r = d.ele_n_i_corr + a * 8;
+        auto key = blank_key(kCorrIon);
+        key[6] = r[2];
+        key[4] = r[4];
+        key[2] = check_cls(r[5]);
+        key[5] = r[6];
+        key[3] = check_cls(r[7]);
+        P.slices[s].push_back({r[0], r[1], r[3], class_of(std::move(key))});
+      }
+  }
+  return P;
+}
+
+static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev, double* v_re, double* v_im,
+                                    hipStream_t st) {
+  PotentialPlan P = plan_potential(d);
+  const int np = P.np, S = P.S;
+  const size_t np2 = (size_t)np * np;
+  // flatten terms slice by slice, grouped by class within a slice
+  std::vector<double> tx, ty, wt;
+  std::vector<PotSeg> segs;
+  std::vector<long> term_off(S + 1, 0), seg_off(S + 1, 0);
+  size_t max_terms = 0;
+  for (int s = 0; s < S; ++s) {
+    auto& v = P.slices[s];
+    std::stable_sort(v.begin(), v.end(), [](const Term& a, const Term& b) { return a.cls < b.cls; });
+    const long base = (long)tx.size();
+    for (size_t i = 0; i < v.size(); ++i) {
+      tx.push_back(v[i].x);
+      ty.push_back(v[i].y);
+      wt.push_back(v[i].w);
+      if (i == 0 || v[i].cls != v[i - 1].cls) segs.push_back({v[i].cls, (int)i, (int)i + 1});
+      else segs.back().end = (int)i + 1;
+    }
+    (void)base;
+    term_off[s + 1] = (long)tx.size();
+    seg_off[s + 1] = (long)segs.size();
+    max_terms = std::max(max_terms, v.size());
+  }
+  DevBuf dtx, dty, dwt, dsegs, dcls, dkx, dky, ds2, dap, dcorr;
+  upload(dtx, tx, st);
+  upload(dty, ty, st);
+  upload(dwt, wt, st);
+  upload(dsegs, segs, st);
+  upload(dcls, P.classes, st);
+  upload(dkx, P.kx1, st);
+  upload(dky, P.ky1, st);
+  ds2.alloc(np2 * sizeof(double));
+  HIP_OK(hipMemcpyAsync(ds2.p, d.s_2, np2 * sizeof(double), hipMemcpyHostToDevice, st));
+  dap.alloc(np2 * sizeof(double));
+  HIP_OK(hipMemcpyAsync(dap.p, d.aper_true, np2 * sizeof(double), hipMemcpyHostToDevice, st));
+  if (d.cim_len > 0) {
+    dcorr.alloc(np2 * d.cim_len * sizeof(double));
+    HIP_OK(hipMemcpyAsync(dcorr.p, d.corr_info_matrix, np2 * d.cim_len * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  DevBuf dex(std::max<size_t>(max_terms, 1) * np * sizeof(float2));
+  DevBuf dey(std::max<size_t>(max_terms, 1) * np * sizeof(float2));
+  tw_table(np);
+  const float sig = (float)(d.sigma / 1000.0);
+  const bool want_v = v_re && v_im;
+  std::vector<float2> hv;
+  if (want_v) hv.resize(np2);
+  for (int s = 0; s < S; ++s) {
+    const int nt = (int)(term_off[s + 1] - term_off[s]);
+    float2* ts = t_dev + (size_t)s * np2;
+    phase_tables(st, dtx.as<double>() + term_off[s], dty.as<double>() + term_off[s], nt, dkx.as<double>(),
+                 dky.as<double>(), np, dex.as<float2>(), dey.as<float2>());
+    accumulate(st, dex.as<float2>(), dey.as<float2>(), dwt.as<double>() + term_off[s],
+               dsegs.as<PotSeg>() + seg_off[s], (int)(seg_off[s + 1] - seg_off[s]), dcls.as<PotClass>(),
+               ds2.as<double>(), dap.as<double>(), dcorr.as<double>(), d.parameter, np, ts);
+    if (want_v) {
+      fft2d(np, st, ts, ts, 1, +1, 0, 0.f);
+      HIP_OK(hipMemcpyAsync(hv.data(), ts, np2 * sizeof(float2), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      for (size_t i = 0; i < np2; ++i) {
+        v_re[(size_t)s * np2 + i] = hv[i].x;
+        v_im[(size_t)s * np2 + i] = hv[i].y;
+        const float amp = std::exp(-sig * hv[i].y);
+        hv[i] = make_float2(amp * std::cos(sig * hv[i].x), amp * std::sin(sig * hv[i].x));
+      }
+      HIP_OK(hipMemcpyAsync(ts, hv.data(), np2 * sizeof(float2), hipMemcpyHostToDevice, st));
+    } else {
+      fft2d(np, st, ts, ts, 1, +1, 1, sig);
+    }
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------- STEM propagation
+
+struct StemWorkspace {
+  float2* psi;
+  double* partial;
+  ItemDev* items;
+  JobDev* job;
+  int* mid_of_slice;
+  int* layer_hit;
+};
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t stem_ws_bytes(int n, int S, int A, int L, int batch) {
+  const int G = detector_groups(n);
+  size_t b = 0;
+  b += align_up((size_t)batch * n * n * sizeof(float2));
+  b += align_up((size_t)batch * (L + 1) * A * G * sizeof(double));
+  b += align_up((size_t)batch * sizeof(ItemDev));
+  b += align_up(sizeof(JobDev));
+  b += align_up((size_t)std::max(S, 1) * sizeof(int));
+  b += align_up((size_t)std::max(L, 1) * sizeof(int));
+  return b;
+}
+
+static StemWorkspace carve(void* base, int n, int S, int A, int L, int batch) {
+  const int G = detector_groups(n);
+  char* p = static_cast<char*>(base);
+  StemWorkspace w;
+  w.psi = reinterpret_cast<float2*>(p);
+  p += align_up((size_t)batch * n * n * sizeof(float2));
+  w.partial = reinterpret_cast<double*>(p);
+  p += align_up((size_t)batch * (L + 1) * A * G * sizeof(double));
+  w.items = reinterpret_cast<ItemDev*>(p);
+  p += align_up((size_t)batch * sizeof(ItemDev));
+  w.job = reinterpret_cast<JobDev*>(p);
+  p += align_up(sizeof(JobDev));
+  w.mid_of_slice = reinterpret_cast<int*>(p);
+  p += align_up((size_t)std::max(S, 1) * sizeof(int));
+  w.layer_hit = reinterpret_cast<int*>(p);
+  return w;
+}
+
+static void validate_job(const totem_stem_job& j) {
+  require(j.n > 0 && j.np > 0 && j.slices > 0, "grid sizes and slice count must be positive");
+  require(j.n % 2 == 0, "probe window side must be even");
+  require(size_supported(j.n), "probe grid " + std::to_string(j.n) + " has no compiled FFT plan",
+          TOTEM_ERR_UNSUPPORTED);
+  require(j.t && j.probes && j.prop && j.aper && j.det, "t, probes, prop, aper and det are required");
+  require(j.n_probes > 0 && j.n_aper > 0 && j.depth > 0 && j.width_red > 0, "w_num, aper_num, Depth, width_red > 0");
+  require(j.n_mid >= 0 && (j.n_mid == 0 || (j.mid && j.aper2)), "mid slices need mid_ceng and aper2");
+  require(j.step >= 0, "step must be non-negative");
+  const long rows = (long)(j.depth - 1) / j.width_red;
+  const long cols = std::min<long>(j.depth, j.width_red) - 1;
+  const long rmax = (long)j.begin_row + rows * j.step, cmax = (long)j.begin_col + cols * j.step;
+  require(j.begin_row >= 0 && j.begin_col >= 0, "beginRow/beginCol must be >= 1 (1-based)");
+  if (j.wrap) {
+    require(j.n <= j.np, "periodic windows need Height <= P_Height");
+  } else {
+    require(rmax + j.n <= j.np && cmax + j.n <= j.np,
+            "probe window leaves the potential: last window ends at row " + std::to_string(rmax + j.n) +
+                ", column " + std::to_string(cmax + j.n) + " of " + std::to_string(j.np));
+  }
+}
+
+static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* ws, size_t ws_bytes, hipStream_t st) {
+  validate_job(j);
+  const int n = j.n, S = j.slices, A = j.n_aper, L = j.n_mid, M = j.n_probes, D = j.depth;
+  const long total = (long)M * D;
+  require(batch > 0, "batch must be positive");
+  batch = (int)std::min<long>(batch, total);
+  const size_t need = stem_ws_bytes(n, S, A, L, batch);
+  DevBuf own;
+  if (!ws) {
+    own.alloc(need);
+    ws = own.p;
+  } else {
+    require(ws_bytes >= need, "workspace too small: need " + std::to_string(need) + " bytes", TOTEM_ERR_MEMORY);
+  }
+  StemWorkspace w = carve(ws, n, S, A, L, batch);
+  const int G = detector_groups(n);
+  tw_table(n);
+  // mid-slice walk exactly as the reference: cuda_source_lobato.cu:1340
+  std::vector<int> mos(S, -1), hit(std::max(L, 1), 0);
+  int li = 0;
+  for (int s = 0; s < S; ++s)
+    if (li < L && s + 1 == j.mid[li]) {
+      mos[s] = li;
+      hit[li] = 1;
+      ++li;
+    }
+  JobDev jd;
+  jd.t = static_cast<const float2*>(j.t);
+  jd.prop = static_cast<const float2*>(j.prop);
+  jd.aper = j.aper;
+  jd.aper2 = j.aper2;
+  jd.mid_of_slice = L > 0 ? w.mid_of_slice : nullptr;
+  jd.layer_hit = w.layer_hit;
+  jd.det = j.det;
+  jd.np = j.np;
+  jd.S = S;
+  jd.A = A;
+  jd.L = L;
+  jd.n_out = (int)total;
+  jd.wrap = j.wrap;
+  HIP_OK(hipMemcpyAsync(w.job, &jd, sizeof(jd), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(w.mid_of_slice, mos.data(), S * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(w.layer_hit, hit.data(), hit.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_OK(hipStreamSynchronize(st));  // host staging buffers above are locals
+  const float2* probes = static_cast<const float2*>(j.probes);
+  std::vector<ItemDev> items(batch);
+  hipGraphExec_t gexec = nullptr;
+  int graph_count = -1;
+  for (long i0 = 0; i0 < total; i0 += batch) {
+    const int cnt = (int)std::min<long>(batch, total - i0);
+    if (i0 > 0) HIP_OK(hipStreamSynchronize(st));  // the previous chunk still reads w.items
+    for (int k = 0; k < cnt; ++k) {
+      const long o = i0 + k;
+      const int m = (int)(o / D), d = (int)(o % D);
+      ItemDev& it = items[k];
+      it.probe = probes + (size_t)m * n * n;
+      it.job = 0;
+      long r0 = (long)j.begin_row + (long)(d / j.width_red) * j.step;
+      long c0 = (long)j.begin_col + (long)(d % j.width_red) * j.step;
+      if (j.wrap) {
+        r0 %= j.np;
+        c0 %= j.np;
+      }
+      it.r0 = (int)r0;
+      it.c0 = (int)c0;
+      it.start_step = 0;
+      it.slot = k;
+      it.out = (int)o;
+    }
+    HIP_OK(hipMemcpyAsync(w.items, items.data(), cnt * sizeof(ItemDev), hipMemcpyHostToDevice, st));
+    auto record = [&]() {
+      for (int s = 0; s < S; ++s) {
+        pass_rows(n, st, w.items, cnt, w.job, w.psi, s);
+        pass_cols(n, st, w.items, cnt, w.job, w.psi, s, w.partial, L + 1, A);
+      }
+      reduce_det(st, w.items, nullptr, cnt, w.job, w.partial, G, L + 1, A);
+    };
+    if (flags & TOTEM_RUN_GRAPH) {
+      if (graph_count != cnt) {
+        if (gexec) HIP_OK(hipGraphExecDestroy(gexec));
+        hipGraph_t g;
+        HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        record();
+        HIP_OK(hipStreamEndCapture(st, &g));
+        HIP_OK(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0));
+        HIP_OK(hipGraphDestroy(g));
+        graph_count = cnt;
+      }
+      HIP_OK(hipGraphLaunch(gexec, st));
+    } else {
+      record();
+    }
+    HIP_OK(hipGetLastError());
+  }
+  if (gexec) {
+    HIP_OK(hipStreamSynchronize(st));
+    HIP_OK(hipGraphExecDestroy(gexec));
+  }
+  if (own.p) HIP_OK(hipStreamSynchronize(st));
+}
+
+// --------------------------------------------------------------------- myProjectfun
+
+struct ProjectArgs {
+  double *my_real_w, *my_imag_w, *aper, *atom_slice, *absorp_n, *atom_slice_i, *absorp_n_i, *proj_coff_mat,
+      *proj_coff_mat_i, *kx, *ky, *s_2, *my_real_const, *my_img_const, *gfsf;
+  int Height, Width, Depth, P_Height, P_Width, Slices, step, beginRow, beginCol, width_red;
+  double sigma;
+  int w_num, aper_num;
+  double parameter;
+  double* AperTrue;
+  int* mid_ceng;
+  int ceng_len;
+  double *aper2, *series_n_corr, *series_n_i_corr, *ele_n_corr, *ele_n_i_corr, *corr_info_matrix;
+  int cim_len;
+  double *return_result, *mid_ceng_mat, *potentialx, *potentialy;
+};
+
+static totem_potential_desc potential_desc(const ProjectArgs& a, int paraflag) {
+  totem_potential_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.paraflag = paraflag;
+  d.np = a.P_Height;
+  d.slices = a.Slices;
+  d.atom_slice = a.atom_slice;
+  d.proj_coff_mat = a.proj_coff_mat;
+  d.absorp_n = a.absorp_n;
+  d.atom_slice_i = a.atom_slice_i;
+  d.proj_coff_mat_i = a.proj_coff_mat_i;
+  d.absorp_n_i = a.absorp_n_i;
+  if (paraflag != 'l' && a.cim_len != 0) {
+    d.series_n_corr = a.series_n_corr;
+    d.ele_n_corr = a.ele_n_corr;
+    d.series_n_i_corr = a.series_n_i_corr;
+    d.ele_n_i_corr = a.ele_n_i_corr;
+    d.corr_info_matrix = a.corr_info_matrix;
+    d.cim_len = a.cim_len;
+  }
+  d.kx = a.kx;
+  d.ky = a.ky;
+  d.s_2 = a.s_2;
+  d.aper_true = a.AperTrue;
+  d.sigma = a.sigma;
+  d.parameter = a.parameter;
+  return d;
+}
+
+static void zero_outputs(const ProjectArgs& a, bool potential_only) {
+  const size_t np2 = (size_t)std::max(a.P_Height, 0) * std::max(a.P_Width, 0);
+  if (!potential_only) {
+    if (a.return_result && a.aper_num > 0 && a.Depth > 0)
+      std::memset(a.return_result, 0, sizeof(double) * a.aper_num * a.Depth);
+    if (a.mid_ceng_mat && a.ceng_len > 0 && a.aper_num > 0 && a.Depth > 0)
+      std::memset(a.mid_ceng_mat, 0, sizeof(double) * a.aper_num * a.Depth * a.ceng_len);
+  } else {
+    if (a.potentialx && a.Slices > 0) std::memset(a.potentialx, 0, sizeof(double) * np2 * a.Slices);
+    if (a.potentialy && a.Slices > 0) std::memset(a.potentialy, 0, sizeof(double) * np2 * a.Slices);
+  }
+}
+
+static void project_impl(const ProjectArgs& a, int paraflag, bool potential_only) {
+  zero_outputs(a, potential_only);
+  require(a.P_Height == a.P_Width, "P_Height must equal P_Width (square potential grid)");
+  require(a.Slices > 0, "Slices must be positive");
+  select_device();
+  hipStream_t st = lib_stream();
+  const int np = a.P_Height, S = a.Slices;
+  const size_t np2 = (size_t)np * np;
+  totem_potential_desc pd = potential_desc(a, paraflag);
+  DevBuf t(np2 * S * sizeof(float2));
+  if (potential_only) {
+    require(a.potentialx && a.potentialy, "potentialx/potentialy outputs are required");
+    build_transmission_impl(pd, t.as<float2>(), nullptr, nullptr, st);
+    std::vector<float2> h(np2 * S);
+    HIP_OK(hipMemcpyAsync(h.data(), t.p, h.size() * sizeof(float2), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (size_t i = 0; i < h.size(); ++i) {
+      a.potentialx[i] = h[i].x;
+      a.potentialy[i] = h[i].y;
+    }
+    return;
+  }
+  require(a.Height == a.Width, "Height must equal Width (square probe window)");
+  require(a.my_real_w && a.my_imag_w && a.aper && a.my_real_const && a.my_img_const && a.gfsf,
+          "probe, aperture, propagator and gfsf arrays are required");
+  require(a.return_result && (a.ceng_len == 0 || a.mid_ceng_mat), "outputs are required");
+  require(a.aper_num > 0 && a.w_num > 0 && a.Depth > 0, "aper_num, w_num and Depth must be positive");
+  const int n = a.Height, M = a.w_num, A = a.aper_num, D = a.Depth, L = std::max(a.ceng_len, 0);
+  const size_t n2 = (size_t)n * n;
+  // Validate the scan geometry before the (expensive) potential build.
+  totem_stem_job j;
+  std::memset(&j, 0, sizeof(j));
+  j.n = n;
+  j.np = np;
+  j.slices = S;
+  j.n_probes = M;
+  j.n_aper = A;
+  j.mid = a.mid_ceng;
+  j.n_mid = L;
+  j.depth = D;
+  j.width_red = a.width_red;
+  j.step = a.step;
+  j.begin_row = a.beginRow;
+  j.begin_col = a.beginCol;
+  j.wrap = 0;
+  {
+    totem_stem_job probe_check = j;
+    probe_check.t = probe_check.probes = probe_check.prop = (const void*)1;
+    probe_check.aper = probe_check.aper2 = (const double*)1;
+    probe_check.det = (double*)1;
+    validate_job(probe_check);
+  }
+  build_transmission_impl(pd, t.as<float2>(), nullptr, nullptr, st);
+  // probes and propagator: double -> complex64 (initKernel_W, cuda_source_lobato.cu:39-57; :1207-1220)
+  std::vector<float2> hp(n2 * M), hh(n2);
+  for (size_t i = 0; i < n2 * M; ++i) hp[i] = make_float2((float)a.my_real_w[i], (float)a.my_imag_w[i]);
+  for (size_t i = 0; i < n2; ++i) hh[i] = make_float2((float)a.my_real_const[i], (float)a.my_img_const[i]);
+  DevBuf dprobe, dprop, daper, daper2, ddet;
+  upload(dprobe, hp, st);
+  upload(dprop, hh, st);
+  daper.alloc(n2 * A * sizeof(double));
+  HIP_OK(hipMemcpyAsync(daper.p, a.aper, n2 * A * sizeof(double), hipMemcpyHostToDevice, st));
+  if (L > 0) {
+    require(a.aper2 != nullptr, "aper2 is required when ceng_len > 0");
+    daper2.alloc(n2 * A * sizeof(double));
+    HIP_OK(hipMemcpyAsync(daper2.p, a.aper2, n2 * A * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  const size_t det_n = (size_t)(L + 1) * A * M * D;
+  ddet.alloc(det_n * sizeof(double));
+  j.t = t.p;
+  j.probes = dprobe.p;
+  j.prop = dprop.p;
+  j.aper = daper.as<double>();
+  j.aper2 = daper2.as<double>();
+  j.det = ddet.as<double>();
+  // probes in flight: bounded by free HBM (the reference sizes its batches the same way,
+  // cuda_source_lobato.cu:1249-1260)
+  size_t fr = 0, tot = 0;
+  HIP_OK(hipMemGetInfo(&fr, &tot));
+  const size_t per = stem_ws_bytes(n, S, A, L, 2) - stem_ws_bytes(n, S, A, L, 1);
+  long batch = (long)(fr / 2 / std::max<size_t>(per, 1));
+  batch = std::max<long>(1, std::min<long>(batch, std::min<long>((long)M * D, 16384)));
+  DevBuf ws(stem_ws_bytes(n, S, A, L, (int)batch));
+  run_stem_impl(j, (int)batch, 0, ws.p, ws.bytes, st);
+  std::vector<double> det(det_n);
+  HIP_OK(hipMemcpyAsync(det.data(), ddet.p, det_n * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  // focal-spread weighting (accumulate_result / accumulate_result2, cuda_source_lobato.cu:981-1015)
+  const size_t MD = (size_t)M * D;
+  for (int aa = 0; aa < A; ++aa)
+    for (int d = 0; d < D; ++d) {
+      double s = 0.0;
+      for (int m = 0; m < M; ++m) s += det[((size_t)L * A + aa) * MD + (size_t)m * D + d] * a.gfsf[m];
+      a.return_result[(size_t)aa * D + d] = s;
+    }
+  for (int l = 0; l < L; ++l)
+    for (int aa = 0; aa < A; ++aa)
+      for (int d = 0; d < D; ++d) {
+        double s = 0.0;
+        for (int m = 0; m < M; ++m) s += det[((size_t)l * A + aa) * MD + (size_t)m * D + d] * a.gfsf[m];
+        a.mid_ceng_mat[((size_t)l * A + aa) * D + d] = s;
+      }
+}
+
+}  // namespace totem
+
+using namespace totem;
+
+#define TOTEM_PACK_ARGS                                                                                         \
+  ProjectArgs a{my_real_w, my_imag_w, aper, atom_slice, absorp_n, atom_slice_i, absorp_n_i, proj_coff_mat,       \
+                proj_coff_mat_i, kx, ky, s_2, my_real_const, my_img_const, gfsf, Height, Width, Depth, P_Height, \
+                P_Width, Slices, step, beginRow, beginCol, width_red, sigma, w_num, aper_num, parameter,         \
+                AperTrue, mid_ceng, ceng_len, aper2, series_n_corr, series_n_i_corr, ele_n_corr, ele_n_i_corr,   \
+                corr_info_matrix, cim_len, return_result, mid_ceng_mat, potentialx, potentialy}
+
+extern "C" {
+
+int totem_project_lobato(TOTEM_MYPROJECTFUN_PARAMS) {
+  TOTEM_PACK_ARGS;
+  return guarded([&] { project_impl(a, 'l', false); });
+}
+int totem_project_peng(TOTEM_MYPROJECTFUN_PARAMS) {
+  TOTEM_PACK_ARGS;
+  return guarded([&] { project_impl(a, 'n', false); });
+}
+int totem_potential_lobato(TOTEM_MYPROJECTFUN_PARAMS) {
+  TOTEM_PACK_ARGS;
+  return guarded([&] { project_impl(a, 'l', true); });
+}
+int totem_potential_peng(TOTEM_MYPROJECTFUN_PARAMS) {
+  TOTEM_PACK_ARGS;
+  return guarded([&] { project_impl(a, 'n', true); });
+}
+
+const char* totem_last_error(void) { return g_last_error.c_str(); }
+int totem_abi_version(void) { return 1; }
+int totem_fft_size_supported(int n) { return size_supported(n) ? 1 : 0; }
+
+int totem_build_transmission(const totem_potential_desc* desc, void* t_dev, double* v_re, double* v_im,
+                             void* stream) {
+  return guarded([&] {
+    require(desc && t_dev, "desc and t_dev are required");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : lib_stream();
+    build_transmission_impl(*desc, static_cast<float2*>(t_dev), v_re, v_im, st);
+  });
+}
+
+size_t totem_stem_workspace_bytes(const totem_stem_job* job, int batch) {
+  if (!job || batch <= 0 || !size_supported(job->n)) return 0;
+  return stem_ws_bytes(job->n, job->slices, job->n_aper, std::max(job->n_mid, 0), batch);
+}
+
+int totem_stem_run(const totem_stem_job* job, int batch, int flags, void* workspace, size_t ws_bytes, void* stream) {
+  return guarded([&] {
+    require(job != nullptr, "job is required");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : lib_stream();
+    run_stem_impl(*job, batch, flags, workspace, ws_bytes, st);
+  });
+}
+
+int totem_fft2d(void* data, int n, int batch, int sign, void* stream) {
+  return guarded([&] {
+    require(data && batch > 0 && (sign == 1 || sign == -1), "bad fft2d arguments");
+    require(size_supported(n), "no FFT plan for n=" + std::to_string(n), TOTEM_ERR_UNSUPPORTED);
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : lib_stream();
+    tw_table(n);
+    fft2d(n, st, static_cast<float2*>(data), static_cast<float2*>(data), batch, sign, 0, 0.f);
+    HIP_OK(hipGetLastError());
+    if (!stream) HIP_OK(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+// Workgroup-level Stockham FFT for CDNA4 (gfx950).
+//
+// A transform of length N is owned by T cooperating lanes; lane t keeps the
+// E = N/T elements  t + T*v  (v = 0..E-1) in VGPRs.  Every Stockham stage of
+// radix R runs E/R butterflies per lane on those registers and, except for the
+// last stage, exchanges through LDS.  Because the register set {t + T*v} is the
+// same at entry and exit, a forward transform can feed an inverse transform (or
+// vice versa) without an LDS round trip, which is what lets the multislice
+// passes fuse  inverse-FFT -> pointwise -> forward-FFT  in one kernel.
+//
+// Twiddles come from an FP32 table tw[m] = exp(-2 pi i m / N) that the host
+// generates in FP64; the inverse direction conjugates them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace totem {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (SIGN * i)
+template <int SIGN> __device__ __forceinline__ float2 mul_si(float2 a) {
+  return SIGN < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+// a * (c + SIGN*i*s)
+template <int SIGN> __device__ __forceinline__ float2 mul_cs(float2 a, float c, float s) {
+  const float ss = SIGN < 0 ? -s : s;
+  return make_float2(a.x * c - a.y * ss, a.x * ss + a.y * c);
+}
+
+// ------------------------------------------------------------------ small DFTs
+// y_k = sum_n a_n exp(SIGN * 2 pi i n k / R), in place, natural order.
+
+template <int SIGN> __device__ __forceinline__ void dft2(float2* a) {
+  float2 t = a[0];
+  a[0] = cadd(t, a[1]);
+  a[1] = csub(t, a[1]);
+}
+
+template <int SIGN> __device__ __forceinline__ void dft3(float2* a) {
+  const float s3 = 0.86602540378443865f;
+  float2 t1 = cadd(a[1], a[2]);
+  float2 t2 = csub(a[1], a[2]);
+  float2 y0 = cadd(a[0], t1);
+  float2 m = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
+  float2 n = mul_si<SIGN>(cscale(t2, s3));
+  a[0] = y0;
+  a[1] = cadd(m, n);
+  a[2] = csub(m, n);
+}
+
+template <int SIGN> __device__ __forceinline__ void dft4(float2* a) {
+  float2 t0 = cadd(a[0], a[2]);
+  float2 t1 = csub(a[0], a[2]);
+  float2 t2 = cadd(a[1], a[3]);
+  float2 t3 = mul_si<SIGN>(csub(a[1], a[3]));
+  a[0] = cadd(t0, t2);
+  a[2] = csub(t0, t2);
+  a[1] = cadd(t1, t3);
+  a[3] = csub(t1, t3);
+}
+
+template <int SIGN> __device__ __forceinline__ void dft5(float2* a) {
+  const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+  const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+  float2 t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+  float2 t3 = csub(a[1], a[4]), t4 = csub(a[2], a[3]);
+  float2 y0 = cadd(a[0], cadd(t1, t2));
+  float2 b1 = make_float2(a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y);
+  float2 b2 = make_float2(a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y);
+  float2 d1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+  float2 d2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+  float2 e1 = mul_si<SIGN>(d1), e2 = mul_si<SIGN>(d2);
+  a[0] = y0;
+  a[1] = cadd(b1, e1);
+  a[4] = csub(b1, e1);
+  a[2] = cadd(b2, e2);
+  a[3] = csub(b2, e2);
+}
+
+template <int SIGN> __device__ __forceinline__ void dft8(float2* a) {
+  const float r = 0.70710678118654752f;
+  float2 e[4] = {a[0], a[2], a[4], a[6]};
+  float2 o[4] = {a[1], a[3], a[5], a[7]};
+  dft4<SIGN>(e);
+  dft4<SIGN>(o);
+  o[1] = mul_cs<SIGN>(o[1], r, r);
+  o[2] = mul_si<SIGN>(o[2]);
+  o[3] = mul_cs<SIGN>(o[3], -r, r);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    a[k] = cadd(e[k], o[k]);
+    a[k + 4] = csub(e[k], o[k]);
+  }
+}
+
+template <int SIGN> __device__ __forceinline__ void dft16(float2* a) {
+  const float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f, r = 0.70710678118654752f;
+  float2 e[8], o[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    e[k] = a[2 * k];
+    o[k] = a[2 * k + 1];
+  }
+  dft8<SIGN>(e);
+  dft8<SIGN>(o);
+  o[1] = mul_cs<SIGN>(o[1], c1, s1);
+  o[2] = mul_cs<SIGN>(o[2], r, r);
+  o[3] = mul_cs<SIGN>(o[3], s1, c1);
+  o[4] = mul_si<SIGN>(o[4]);
+  o[5] = mul_cs<SIGN>(o[5], -s1, c1);
+  o[6] = mul_cs<SIGN>(o[6], -r, r);
+  o[7] = mul_cs<SIGN>(o[7], -c1, s1);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    a[k] = cadd(e[k], o[k]);
+    a[k + 8] = csub(e[k], o[k]);
+  }
+}
+
+template <int R, int SIGN> __device__ __forceinline__ void dft(float2* a) {
+  if constexpr (R == 2) dft2<SIGN>(a);
+  else if constexpr (R == 3) dft3<SIGN>(a);
+  else if constexpr (R == 4) dft4<SIGN>(a);
+  else if constexpr (R == 5) dft5<SIGN>(a);
+  else if constexpr (R == 8) dft8<SIGN>(a);
+  else if constexpr (R == 16) dft16<SIGN>(a);
+  else static_assert(R == 2, "unsupported radix");
+}
+
+// ------------------------------------------------------------------ Stockham stages
+//
+// Stage with radix R after radices of product P (P = 1 for the first stage):
+// butterfly b in [0, N/R) reads x[b + q*N/R], multiplies input q by
+// W^(q*k) with k = b mod P and W = exp(SIGN 2 pi i / (P R)), runs DFT_R and
+// writes y[(b - k)*R + k + r*P].  Lane t runs butterflies b = t + T*u.
+
+template <int N, int T, int SIGN, class Lay, int P, int... RS> struct Stages;
+
+template <int N, int T, int SIGN, class Lay, int P> struct Stages<N, T, SIGN, Lay, P> {
+  static __device__ __forceinline__ void run(float2*, float2*, const Lay&, int, const float2*) {}
+};
+
+template <int N, int T, int SIGN, class Lay, int P, int R, int... RS> struct Stages<N, T, SIGN, Lay, P, R, RS...> {
+  static constexpr int E = N / T;
+  static constexpr int U = E / R;
+  static_assert(E % R == 0, "radix must divide elements per lane");
+  static constexpr bool LAST = sizeof...(RS) == 0;
+
+  static __device__ __forceinline__ void run(float2* x, float2* lds, const Lay& lay, int t, const float2* tw) {
+    constexpr int STRIDE = N / (P * R);  // twiddle table stride
+    float2 out[E];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = t + T * u;
+      const int k = (P == 1) ? 0 : (b % P);
+      float2 a[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) a[q] = x[u + q * U];
+      if constexpr (P > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          float2 w = tw[q * k * STRIDE];
+          if (SIGN > 0) w.y = -w.y;
+          a[q] = cmul(a[q], w);
+        }
+      }
+      dft<R, SIGN>(a);
+      if constexpr (LAST) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[u + r * U] = a[r];
+      } else {
+        const int j = (b - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lay(j + r * P)] = a[r];
+      }
+    }
+    if constexpr (LAST) {
+#pragma unroll
+      for (int v = 0; v < E; ++v) x[v] = out[v];
+    } else {
+      __syncthreads();
+#pragma unroll
+      for (int v = 0; v < E; ++v) x[v] = lds[lay(t + T * v)];
+      __syncthreads();
+      Stages<N, T, SIGN, Lay, P * R, RS...>::run(x, lds, lay, t, tw);
+    }
+  }
+};
+
+// Radix plans: lane count T and radix list per supported length.
+template <int N> struct Plan;
+template <> struct Plan<64> { static constexpr int T = 16; template <int S, class L> using St = Stages<64, 16, S, L, 1, 4, 4, 4>; };
+template <> struct Plan<128> { static constexpr int T = 16; template <int S, class L> using St = Stages<128, 16, S, L, 1, 8, 4, 4>; };
+template <> struct Plan<256> { static constexpr int T = 16; template <int S, class L> using St = Stages<256, 16, S, L, 1, 16, 16>; };
+template <> struct Plan<512> { static constexpr int T = 32; template <int S, class L> using St = Stages<512, 32, S, L, 1, 16, 16, 2>; };
+template <> struct Plan<768> { static constexpr int T = 32; template <int S, class L> using St = Stages<768, 32, S, L, 1, 8, 3, 8, 4>; };
+template <> struct Plan<1024> { static constexpr int T = 64; template <int S, class L> using St = Stages<1024, 64, S, L, 1, 16, 16, 4>; };
+template <> struct Plan<1536> { static constexpr int T = 64; template <int S, class L> using St = Stages<1536, 64, S, L, 1, 8, 3, 8, 8>; };
+template <> struct Plan<2048> { static constexpr int T = 128; template <int S, class L> using St = Stages<2048, 128, S, L, 1, 16, 16, 8>; };
+
+// x holds elements t + T*v of one transform; after the call it holds the
+// corresponding outputs.
+template <int N, int SIGN, class Lay>
+__device__ __forceinline__ void block_fft(float2* x, float2* lds, const Lay& lay, int t, const float2* tw) {
+  Plan<N>::template St<SIGN, Lay>::run(x, lds, lay, t, tw);
+}
+
+// LDS layouts.  Row transforms: one padding slot per 16 elements so that the
+// first stage's stride-R writes spread over the banks.  Column transforms:
+// C interleaved columns, element e of column c at e*C + c (conflict-free
+// because the C lanes of a group touch consecutive slots).
+struct RowLay {
+  int base;
+  __device__ __forceinline__ int operator()(int e) const { return base + e + (e >> 4); }
+};
+template <int C> struct ColLay {
+  int c;
+  __device__ __forceinline__ int operator()(int e) const { return e * C + c; }
+};
+template <int N> __host__ __device__ constexpr int row_lds_len() { return N + N / 16; }
+
+}  // namespace totem

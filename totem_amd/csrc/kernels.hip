@@ -1,0 +1,452 @@
+// Hot-path kernels of the multislice STEM engine (gfx950).
+//
+//  * k_pass_rows  -- per slice, along rows:  [inverse row FFT, 1/N^2] -> x t(window) -> forward row FFT
+//  * k_pass_cols  -- per slice, along columns: forward column FFT -> x propagator ->
+//                    [annular detectors] -> inverse column FFT   (exit slice: detectors only)
+//  * k_reduce_det -- per finished probe: fixed-order compensated FP64 sum of detector partials
+//  * k_phase / k_accumulate -- projected potential in reciprocal space per slice
+//  * k_fft_rows / k_fft_cols -- plain batched 2-D FFT passes (potential IFFT, tests)
+//
+// A full 2-D forward+inverse FFT per slice therefore costs two read+write
+// passes over the probe wave in HBM (32 N^2 bytes per probe.slice).
+#include "fft_block.h"
+#include "kernels.h"
+#include "launch.h"
+
+namespace totem {
+
+// ------------------------------------------------------------------------- helpers
+
+template <int N> struct RowCfg {
+  static constexpr int T = Plan<N>::T;
+  static constexpr int RW = (256 / T) > 0 ? 256 / T : 1;  // rows per workgroup
+  static constexpr int THREADS = RW * T;
+  static constexpr int LL = row_lds_len<N>();
+};
+template <int N> struct ColCfg {
+  static constexpr int T = Plan<N>::T;
+  static constexpr int C = (N <= 512) ? 16 : (N >= 1536 ? 4 : 8);  // columns per workgroup
+  static constexpr int THREADS = T * C;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ------------------------------------------------------------------------ pass R
+
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS)
+k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, float2* __restrict__ psi,
+            int step, const float2* __restrict__ tw) {
+  using Cfg = RowCfg<N>;
+  constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, G = N / RW;
+  __shared__ float2 lds[RW * Cfg::LL];
+  const int item = blockIdx.x / G;
+  const int rg = blockIdx.x - item * G;
+  const ItemDev it = items[item];
+  const JobDev& jb = jobs[it.job];
+  const int slice = step - it.start_step;
+  if (slice < 0 || slice >= jb.S) return;  // uniform over the workgroup
+  const int lr = threadIdx.x / T;
+  const int t = threadIdx.x - lr * T;
+  const int row = rg * RW + lr;
+  const RowLay lay{lr * Cfg::LL};
+  float2 x[E];
+  float2* w = psi + (size_t)it.slot * N * N + (size_t)row * N;
+  if (slice == 0) {
+    const float2* p = it.probe + (size_t)row * N;
+#pragma unroll
+    for (int v = 0; v < E; ++v) x[v] = p[t + T * v];
+  } else {
+#pragma unroll
+    for (int v = 0; v < E; ++v) x[v] = w[t + T * v];
+    block_fft<N, +1>(x, lds, lay, t, tw);
+    const float s = 1.0f / (float)(N * N);
+#pragma unroll
+    for (int v = 0; v < E; ++v) x[v] = cscale(x[v], s);
+  }
+  const int np = jb.np;
+  int pr = it.r0 + row;
+  if (pr >= np) pr -= np;  // only reachable with wrap (validated on the host)
+  const float2* trow = jb.t + (size_t)slice * np * np + (size_t)pr * np;
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    int pc = it.c0 + t + T * v;
+    if (pc >= np) pc -= np;
+    x[v] = cmul(x[v], trow[pc]);
+  }
+  block_fft<N, -1>(x, lds, lay, t, tw);
+#pragma unroll
+  for (int v = 0; v < E; ++v) w[t + T * v] = x[v];
+}
+
+// ------------------------------------------------------------------------ pass C
+
+template <int N, int E, int T, int C>
+__device__ __forceinline__ void detector_partial(const float2* x, const double* __restrict__ masks, bool centred,
+                                                 int A, int t, int col, double* red, double* out, int G, int cg) {
+  // sum_k |Psi(k)|^2 * mask(k) over this workgroup's columns, FP64
+  double acc[kMaxAper];
+#pragma unroll
+  for (int a = 0; a < kMaxAper; ++a) acc[a] = 0.0;
+  const int mcol = centred ? ((col + N / 2) % N) : col;
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    const int row = t + T * v;
+    const int mrow = centred ? ((row + N / 2) % N) : row;
+    const double inten = (double)x[v].x * (double)x[v].x + (double)x[v].y * (double)x[v].y;
+    const double* m = masks + (size_t)mrow * N + mcol;
+#pragma unroll
+    for (int a = 0; a < kMaxAper; ++a)
+      if (a < A) acc[a] += inten * m[(size_t)a * N * N];
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int NW = (T * C + 63) / 64;
+#pragma unroll
+  for (int a = 0; a < kMaxAper; ++a) {
+    if (a < A) {
+      double s = wave_sum(acc[a]);
+      if (lane == 0) red[wave * kMaxAper + a] = s;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < A) {
+    double s = 0.0;
+    for (int q = 0; q < NW; ++q) s += red[q * kMaxAper + threadIdx.x];
+    out[(size_t)threadIdx.x * G + cg] = s;
+  }
+  __syncthreads();
+}
+
+template <int N>
+__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, float2* __restrict__ psi,
+            int step, const float2* __restrict__ tw, double* __restrict__ partial, int lmax1, int amax) {
+  using Cfg = ColCfg<N>;
+  constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C;
+  constexpr int NW = (Cfg::THREADS + 63) / 64;
+  __shared__ float2 lds[N * C];
+  __shared__ double red[NW * kMaxAper];
+  const int item = blockIdx.x / G;
+  const int cg = blockIdx.x - item * G;
+  const ItemDev it = items[item];
+  const JobDev& jb = jobs[it.job];
+  const int slice = step - it.start_step;
+  if (slice < 0 || slice >= jb.S) return;
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int col = cg * C + c;
+  const ColLay<C> lay{c};
+  float2 x[E];
+  float2* w = psi + (size_t)it.slot * N * N + col;
+#pragma unroll
+  for (int v = 0; v < E; ++v) x[v] = w[(size_t)(t + T * v) * N];
+  block_fft<N, -1>(x, lds, lay, t, tw);
+  const float2* hp = jb.prop + col;
+#pragma unroll
+  for (int v = 0; v < E; ++v) x[v] = cmul(x[v], hp[(size_t)(t + T * v) * N]);
+  const int mid = jb.mid_of_slice ? jb.mid_of_slice[slice] : -1;
+  double* pbase = partial + (size_t)it.slot * lmax1 * amax * G;
+  if (mid >= 0) {
+    for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
+      detector_partial<N, E, T, C>(x, jb.aper2 + (size_t)a0 * N * N, false, min(kMaxAper, jb.A - a0), t, col, red,
+                                   pbase + ((size_t)mid * amax + a0) * G, G, cg);
+  }
+  if (slice == jb.S - 1) {
+    for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
+      detector_partial<N, E, T, C>(x, jb.aper + (size_t)a0 * N * N, true, min(kMaxAper, jb.A - a0), t, col, red,
+                                   pbase + ((size_t)jb.L * amax + a0) * G, G, cg);
+    return;
+  }
+  block_fft<N, +1>(x, lds, lay, t, tw);
+#pragma unroll
+  for (int v = 0; v < E; ++v) w[(size_t)(t + T * v) * N] = x[v];
+}
+
+// Fixed-order Kahan sum of the detector partials of the listed items.
+__global__ void k_reduce_det(const ItemDev* __restrict__ items, const int* __restrict__ list, int n,
+                             const JobDev* __restrict__ jobs, const double* __restrict__ partial, int G,
+                             int lmax1, int amax) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * lmax1 * amax) return;
+  const int a = idx % amax;
+  const int layer = (idx / amax) % lmax1;
+  const int e = idx / (amax * lmax1);
+  const ItemDev it = items[list ? list[e] : e];
+  const JobDev& jb = jobs[it.job];
+  if (a >= jb.A || layer > jb.L) return;
+  double s = 0.0;
+  if (layer == jb.L || jb.layer_hit[layer]) {
+    const double* p = partial + (((size_t)it.slot * lmax1 + layer) * amax + a) * G;
+    double comp = 0.0;
+    for (int g = 0; g < G; ++g) {
+      const double y = p[g] - comp;
+      const double tt = s + y;
+      comp = (tt - s) - y;
+      s = tt;
+    }
+  }
+  jb.det[((size_t)layer * jb.A + a) * jb.n_out + it.out] = s;
+}
+
+// --------------------------------------------------------------- plain 2-D FFT passes
+
+template <int N, int SIGN>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS) k_fft_rows(float2* __restrict__ data, long rows, const float2* __restrict__ tw) {
+  using Cfg = RowCfg<N>;
+  constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T;
+  __shared__ float2 lds[RW * Cfg::LL];
+  const int lr = threadIdx.x / T;
+  const int t = threadIdx.x - lr * T;
+  const long row = (long)blockIdx.x * RW + lr;
+  float2* w = data + row * N;
+  float2 x[E];
+  const bool ok = row < rows;
+#pragma unroll
+  for (int v = 0; v < E; ++v) x[v] = ok ? w[t + T * v] : make_float2(0.f, 0.f);
+  block_fft<N, SIGN>(x, lds, RowLay{lr * Cfg::LL}, t, tw);
+  if (ok) {
+#pragma unroll
+    for (int v = 0; v < E; ++v) w[t + T * v] = x[v];
+  }
+}
+
+// epi: 0 store, 1 store exp(i*sigma*V) (transmission), 2 store scaled by 1/N^2
+template <int N, int SIGN>
+__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+k_fft_cols(const float2* __restrict__ src, float2* __restrict__ dst, int epi, float sigma,
+           const float2* __restrict__ tw) {
+  using Cfg = ColCfg<N>;
+  constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C;
+  __shared__ float2 lds[N * C];
+  const int mat = blockIdx.x / G;
+  const int cg = blockIdx.x - mat * G;
+  const int c = threadIdx.x % C;
+  const int t = threadIdx.x / C;
+  const int col = cg * C + c;
+  const float2* s = src + (size_t)mat * N * N + col;
+  float2* d = dst + (size_t)mat * N * N + col;
+  float2 x[E];
+#pragma unroll
+  for (int v = 0; v < E; ++v) x[v] = s[(size_t)(t + T * v) * N];
+  block_fft<N, SIGN>(x, lds, ColLay<C>{c}, t, tw);
+#pragma unroll
+  for (int v = 0; v < E; ++v) {
+    float2 y = x[v];
+    if (epi == 1) {
+      // t = exp(i sigma V) = exp(-sigma Im V) (cos(sigma Re V) + i sin(sigma Re V))
+      const float amp = expf(-sigma * y.y);
+      float sn, cs;
+      sincosf(sigma * y.x, &sn, &cs);
+      y = make_float2(amp * cs, amp * sn);
+    } else if (epi == 2) {
+      y = cscale(y, 1.0f / (float)((long)N * N));
+    }
+    d[(size_t)(t + T * v) * N] = y;
+  }
+}
+
+// ------------------------------------------------------------------- potential build
+
+// ex[j][c] = exp(-2 pi i kx1[c] x_j), ey[j][r] = exp(-2 pi i ky1[r] y_j); the product
+// k.x is formed and reduced modulo 1 in FP64 before the (FP64) sincospi.
+__global__ void k_phase(const double* __restrict__ tx, const double* __restrict__ ty, int n_terms,
+                        const double* __restrict__ kx1, const double* __restrict__ ky1, int np,
+                        float2* __restrict__ ex, float2* __restrict__ ey) {
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
+  const int j = blockIdx.x;
+  if (c >= np || j >= n_terms) return;
+  double fx = kx1[c] * tx[j];
+  fx -= rint(fx);
+  double fy = ky1[c] * ty[j];
+  fy -= rint(fy);
+  double sx, cx, sy, cy;
+  sincospi(2.0 * fx, &sx, &cx);
+  sincospi(2.0 * fy, &sy, &cy);
+  ex[(size_t)j * np + c] = make_float2((float)cx, (float)-sx);
+  ey[(size_t)j * np + c] = make_float2((float)cy, (float)-sy);
+}
+
+__device__ __forceinline__ double peng5(const double* p, double s2) {
+  double f = 0.0;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) f += p[2 * q] * exp(-p[2 * q + 1] * s2);
+  return f;
+}
+__device__ __forceinline__ double lobato5(const double* p, double g2) {
+  double f = 0.0;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double d = 1.0 + p[2 * q + 1] * g2;
+    f += p[2 * q] * (2.0 + p[2 * q + 1] * g2) / (d * d);
+  }
+  return f;
+}
+
+// Form factor of a class at one reciprocal-space pixel (FP64, complex when absorptive).
+__device__ __forceinline__ double2 form_factor(const PotClass& k, double s2, const double* __restrict__ corr,
+                                               size_t pix, size_t np2) {
+  double re = 0.0;
+  switch (k.kind) {
+    case kPeng: re = peng5(k.p1, s2); break;
+    case kLobato: re = lobato5(k.p1, 4.0 * s2) * exp(-k.B * s2); break;
+    case kPengIon: re = k.w1 * peng5(k.p1, s2) + k.w2 * peng5(k.p2, s2); break;
+    case kLobatoIon: re = (k.w1 * lobato5(k.p1, 4.0 * s2) + k.w2 * lobato5(k.p2, 4.0 * s2)) * exp(-k.B * s2); break;
+    case kCorr: re = corr[(size_t)k.i1 * np2 + pix] * exp(-k.B * s2); break;
+    case kCorrIon:
+      re = (k.w1 * corr[(size_t)k.i1 * np2 + pix] + k.w2 * corr[(size_t)k.i2 * np2 + pix]) * exp(-k.B * s2);
+      break;
+    default: break;
+  }
+  const double im = k.has_abs ? peng5(k.ab, s2) : 0.0;
+  return make_double2(re, im);
+}
+
+// One slice: V_k(r,c) = PARAMETER * AperTrue * sum_classes f_cls(k) * sum_{j in cls} w_j ex_j[c] ey_j[r],
+// written at the ifftshifted position (the reference's quadrant swap) as complex64.
+constexpr int kAccTile = 16;
+constexpr int kAccChunk = 64;
+__global__ void __launch_bounds__(256)
+k_accumulate(const float2* __restrict__ ex, const float2* __restrict__ ey, const double* __restrict__ wt,
+             const PotSeg* __restrict__ segs, int n_segs, const PotClass* __restrict__ classes,
+             const double* __restrict__ s2, const double* __restrict__ aper_true, const double* __restrict__ corr,
+             double parameter, int np, float2* __restrict__ vshift) {
+  __shared__ float2 sx[kAccChunk][kAccTile];
+  __shared__ float2 sy[kAccChunk][kAccTile];
+  __shared__ float sw[kAccChunk];
+  const int lc = threadIdx.x % kAccTile, lr = threadIdx.x / kAccTile;
+  const int c0 = blockIdx.x * kAccTile, r0 = blockIdx.y * kAccTile;
+  const int c = c0 + lc, r = r0 + lr;
+  const bool inside = (r < np) && (c < np);
+  const size_t pix = (size_t)r * np + c;
+  const size_t np2 = (size_t)np * np;
+  const double s2v = inside ? s2[pix] : 0.0;
+  double vr = 0.0, vi = 0.0;
+  for (int sgi = 0; sgi < n_segs; ++sgi) {
+    const PotSeg sg = segs[sgi];
+    float ar = 0.f, ai = 0.f;
+    for (int base = sg.begin; base < sg.end; base += kAccChunk) {
+      const int cnt = min(kAccChunk, sg.end - base);
+      __syncthreads();
+      for (int i = threadIdx.x; i < kAccChunk * kAccTile; i += blockDim.x) {
+        const int jj = i / kAccTile, q = i % kAccTile;
+        if (jj < cnt) {
+          const size_t row = (size_t)(base + jj) * np;
+          sx[jj][q] = (c0 + q < np) ? ex[row + c0 + q] : make_float2(0.f, 0.f);
+          sy[jj][q] = (r0 + q < np) ? ey[row + r0 + q] : make_float2(0.f, 0.f);
+        }
+      }
+      for (int i = threadIdx.x; i < cnt; i += blockDim.x) sw[i] = (float)wt[base + i];
+      __syncthreads();
+      for (int jj = 0; jj < cnt; ++jj) {
+        const float2 p = cmul(sx[jj][lc], sy[jj][lr]);
+        ar = fmaf(sw[jj], p.x, ar);
+        ai = fmaf(sw[jj], p.y, ai);
+      }
+    }
+    if (inside) {
+      const double2 f = form_factor(classes[sg.cls], s2v, corr, pix, np2);
+      vr += f.x * ar - f.y * ai;
+      vi += f.x * ai + f.y * ar;
+    }
+  }
+  if (inside) {
+    const double scale = parameter * aper_true[pix];
+    const int half = np / 2;
+    const int rs = (r + half) % np, cs = (c + half) % np;
+    vshift[(size_t)rs * np + cs] = make_float2((float)(vr * scale), (float)(vi * scale));
+  }
+}
+
+// --------------------------------------------------------------------- launchers
+
+template <int N> void launch_pass_rows(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
+                                       float2* psi, int step) {
+  using Cfg = RowCfg<N>;
+  const dim3 grid((unsigned)(n_items * (N / Cfg::RW)));
+  hipLaunchKernelGGL(k_pass_rows<N>, grid, dim3(Cfg::THREADS), 0, st, items, jobs, psi, step, tw_table(N));
+}
+template <int N> void launch_pass_cols(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
+                                       float2* psi, int step, double* partial, int lmax1, int amax) {
+  using Cfg = ColCfg<N>;
+  const dim3 grid((unsigned)(n_items * (N / Cfg::C)));
+  hipLaunchKernelGGL(k_pass_cols<N>, grid, dim3(Cfg::THREADS), 0, st, items, jobs, psi, step, tw_table(N),
+                     partial, lmax1, amax);
+}
+template <int N> void launch_fft2d(hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi,
+                                   float sigma) {
+  using R = RowCfg<N>;
+  using Cc = ColCfg<N>;
+  const long rows = (long)batch * N;
+  const dim3 rgrid((unsigned)((rows + R::RW - 1) / R::RW));
+  const float2* tw = tw_table(N);
+  if (sign < 0) hipLaunchKernelGGL((k_fft_rows<N, -1>), rgrid, dim3(R::THREADS), 0, st, data, rows, tw);
+  else hipLaunchKernelGGL((k_fft_rows<N, +1>), rgrid, dim3(R::THREADS), 0, st, data, rows, tw);
+  const dim3 cgrid((unsigned)(batch * (N / Cc::C)));
+  if (sign < 0) hipLaunchKernelGGL((k_fft_cols<N, -1>), cgrid, dim3(Cc::THREADS), 0, st, data, dst, epi, sigma, tw);
+  else hipLaunchKernelGGL((k_fft_cols<N, +1>), cgrid, dim3(Cc::THREADS), 0, st, data, dst, epi, sigma, tw);
+}
+template <int N> int col_groups() { return N / ColCfg<N>::C; }
+
+#define TOTEM_FOR_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(1536) X(2048)
+
+bool size_supported(int n) {
+#define X(S) if (n == S) return true;
+  TOTEM_FOR_SIZES(X)
+#undef X
+  return false;
+}
+
+int detector_groups(int n) {
+#define X(S) if (n == S) return col_groups<S>();
+  TOTEM_FOR_SIZES(X)
+#undef X
+  return 0;
+}
+
+void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step) {
+#define X(S) if (n == S) { launch_pass_rows<S>(st, items, n_items, jobs, psi, step); return; }
+  TOTEM_FOR_SIZES(X)
+#undef X
+}
+
+void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step,
+               double* partial, int lmax1, int amax) {
+#define X(S) if (n == S) { launch_pass_cols<S>(st, items, n_items, jobs, psi, step, partial, lmax1, amax); return; }
+  TOTEM_FOR_SIZES(X)
+#undef X
+}
+
+void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs,
+                const double* partial, int groups, int lmax1, int amax) {
+  const int total = n * lmax1 * amax;
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_reduce_det, dim3((total + 255) / 256), dim3(256), 0, st, items, list, n, jobs, partial,
+                     groups, lmax1, amax);
+}
+
+void fft2d(int n, hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi, float sigma) {
+#define X(S) if (n == S) { launch_fft2d<S>(st, data, dst, batch, sign, epi, sigma); return; }
+  TOTEM_FOR_SIZES(X)
+#undef X
+}
+
+void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_terms, const double* kx1,
+                  const double* ky1, int np, float2* ex, float2* ey) {
+  if (n_terms <= 0) return;
+  hipLaunchKernelGGL(k_phase, dim3(n_terms, (np + 255) / 256), dim3(256), 0, st, tx, ty, n_terms, kx1, ky1, np,
+                     ex, ey);
+}
+
+void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,
+                const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
+                double parameter, int np, float2* vshift) {
+  const dim3 grid((np + kAccTile - 1) / kAccTile, (np + kAccTile - 1) / kAccTile);
+  hipLaunchKernelGGL(k_accumulate, grid, dim3(256), 0, st, ex, ey, wt, segs, n_segs, classes, s2, aper_true, corr,
+                     parameter, np, vshift);
+}
+
+}  // namespace totem

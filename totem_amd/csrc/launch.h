@@ -1,0 +1,29 @@
+// Host-side launch interface of the kernels in kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+
+namespace totem {
+
+// FP32 twiddle table exp(-2 pi i m / n), m < n, generated in FP64; device
+// pointer for the current device, created on first use and kept for the process.
+const float2* tw_table(int n);
+
+bool size_supported(int n);   // FFT lengths with a compiled radix plan
+int detector_groups(int n);   // column groups per probe in k_pass_cols
+
+void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step);
+void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step,
+               double* partial, int lmax1, int amax);
+void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs,
+                const double* partial, int groups, int lmax1, int amax);
+// In-place batched 2-D FFT of `batch` n x n complex64 matrices; the column pass
+// writes to dst (may equal data).  epi: 0 plain, 1 transmission exp(i sigma V), 2 scale 1/n^2.
+void fft2d(int n, hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi, float sigma);
+void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_terms, const double* kx1,
+                  const double* ky1, int np, float2* ex, float2* ey);
+void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,
+                const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
+                double parameter, int np, float2* vshift);
+
+}  // namespace totem

@@ -1,0 +1,275 @@
+"""ctypes binding of the engine's C ABI (include/totem_abi.h).
+
+Mirrors the reference's mex call shape:
+
+    [myresul, mid_ceng_mat, potentialx, potentialy] = waveCudalobato_<v>(41 args)
+
+(``waveCuda.cpp:4-101``, driven from ``cuda_stem.m:81-107``) as
+:func:`wave_cuda`, with numpy arrays in the row-major layout that
+``cuda_stem.m:22-79`` hands to the mex.  The library is the in-tree
+``totem_amd/lib/libtotem.so``; importing this module on a machine without it
+raises immediately (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+CORE = os.path.join(LIB_DIR, "libtotem.so")
+VARIANT_LIBS = {
+    "waveCudalobato": "libwaveCudalobato.so",
+    "waveCudaPengAndCorr": "libwaveCudaPengAndCorr.so",
+    "waveCudalobatoP": "libwaveCudalobatoP.so",
+    "waveCudaPengAndCorrP": "libwaveCudaPengAndCorrP.so",
+}
+ENTRY = {
+    "waveCudalobato": "totem_project_lobato",
+    "waveCudaPengAndCorr": "totem_project_peng",
+    "waveCudalobatoP": "totem_potential_lobato",
+    "waveCudaPengAndCorrP": "totem_potential_peng",
+}
+# the symbols include/totem_abi.h declares
+EXPORTS = (
+    "myProjectfun",
+    "totem_project_lobato", "totem_project_peng", "totem_potential_lobato", "totem_potential_peng",
+    "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
+    "totem_stem_workspace_bytes", "totem_stem_run", "totem_fft2d",
+)
+
+TOTEM_RUN_GRAPH = 1
+
+
+class TotemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("totem error %d: %s" % (code, msg))
+        self.code = code
+
+
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_VP = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+# the 43 parameters of myProjectfun, in prototype order
+_PROJECT_ARGTYPES = [_DP] * 15 + [_I] * 10 + [_D, _I, _I, _D, _DP, _IP, _I] + [_DP] * 6 + [_I] + [_DP] * 4
+
+
+class PotentialDesc(ctypes.Structure):
+    _fields_ = [
+        ("paraflag", _I), ("np", _I), ("slices", _I),
+        ("atom_slice", _DP), ("proj_coff_mat", _DP), ("absorp_n", _DP),
+        ("atom_slice_i", _DP), ("proj_coff_mat_i", _DP), ("absorp_n_i", _DP),
+        ("series_n_corr", _DP), ("ele_n_corr", _DP), ("series_n_i_corr", _DP), ("ele_n_i_corr", _DP),
+        ("corr_info_matrix", _DP), ("cim_len", _I),
+        ("kx", _DP), ("ky", _DP), ("s_2", _DP), ("aper_true", _DP),
+        ("sigma", _D), ("parameter", _D),
+    ]
+
+
+class StemJob(ctypes.Structure):
+    _fields_ = [
+        ("n", _I), ("np", _I), ("slices", _I),
+        ("t", _VP), ("probes", _VP), ("n_probes", _I), ("prop", _VP),
+        ("aper", _VP), ("aper2", _VP), ("n_aper", _I),
+        ("mid", _IP), ("n_mid", _I),
+        ("depth", _I), ("width_red", _I), ("step", _I), ("begin_row", _I), ("begin_col", _I), ("wrap", _I),
+        ("det", _VP),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libtotem.so (in-tree build).  Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        try:
+            # torch ships its own HIP runtime; importing it first makes the engine library bind to
+            # that same runtime (one runtime per process, shared device pointers and streams)
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        if not os.path.exists(CORE):
+            raise RuntimeError("totem engine library not built: %s (run __graft_entry__.build())" % CORE)
+        L = ctypes.CDLL(CORE, mode=ctypes.RTLD_GLOBAL)
+        for name in ENTRY.values():
+            getattr(L, name).argtypes = _PROJECT_ARGTYPES
+            getattr(L, name).restype = _I
+        L.totem_last_error.restype = ctypes.c_char_p
+        L.totem_abi_version.restype = _I
+        L.totem_fft_size_supported.argtypes = [_I]
+        L.totem_fft_size_supported.restype = _I
+        L.totem_build_transmission.argtypes = [ctypes.POINTER(PotentialDesc), _VP, _DP, _DP, _VP]
+        L.totem_build_transmission.restype = _I
+        L.totem_stem_workspace_bytes.argtypes = [ctypes.POINTER(StemJob), _I]
+        L.totem_stem_workspace_bytes.restype = ctypes.c_size_t
+        L.totem_stem_run.argtypes = [ctypes.POINTER(StemJob), _I, _I, _VP, ctypes.c_size_t, _VP]
+        L.totem_stem_run.restype = _I
+        L.totem_fft2d.argtypes = [_VP, _I, _I, _I, _VP]
+        L.totem_fft2d.restype = _I
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise TotemError(rc, lib().totem_last_error().decode())
+
+
+def _dp(a):
+    """double* view of a C-contiguous float64 array (or NULL)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, (a.dtype, a.flags)
+    return a.ctypes.data_as(_DP)
+
+
+def _f64(a):
+    if a is None:
+        return None
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def my_project_fun(entry, my_real_w, my_imag_w, aper, atom_slice, absorp_n, atom_slice_i, absorp_n_i,
+                   proj_coff_mat, proj_coff_mat_i, kx, ky, s_2, my_real_const, my_img_const, gfsf,
+                   Height, Width, Depth, P_Height, P_Width, Slices, step, beginRow, beginCol, width_red,
+                   sigma, w_num, aper_num, parameter, AperTrue, mid_ceng, ceng_len, aper2,
+                   series_n_corr, series_n_i_corr, ele_n_corr, ele_n_i_corr, corr_info_matrix, cim_len):
+    """Call one engine entry point with the 39 inputs of myProjectfun (``waveCuda.h:10-21``);
+    beginRow/beginCol are 0-based here, as in the C prototype.  Returns
+    (return_result [aper_num, Depth], mid_ceng_mat [ceng_len, aper_num, Depth],
+    potentialx, potentialy [Slices, P_Height, P_Width])."""
+    arrays = [_f64(v) for v in (my_real_w, my_imag_w, aper, atom_slice, absorp_n, atom_slice_i, absorp_n_i,
+                                proj_coff_mat, proj_coff_mat_i, kx, ky, s_2, my_real_const, my_img_const, gfsf)]
+    ap_true = _f64(AperTrue)
+    mid = np.ascontiguousarray(np.asarray(mid_ceng if mid_ceng is not None else [], dtype=np.int32))
+    tail = [_f64(v) for v in (aper2, series_n_corr, series_n_i_corr, ele_n_corr, ele_n_i_corr, corr_info_matrix)]
+    res = np.zeros((aper_num, Depth))
+    midm = np.zeros((max(ceng_len, 0), aper_num, Depth))
+    potential_only = entry.startswith("totem_potential")
+    np2 = P_Height * P_Width
+    px = np.zeros((Slices, P_Height, P_Width)) if potential_only else np.zeros(1)
+    py = np.zeros((Slices, P_Height, P_Width)) if potential_only else np.zeros(1)
+    fn = getattr(lib(), entry)
+    rc = fn(*[_dp(a) for a in arrays], int(Height), int(Width), int(Depth), int(P_Height), int(P_Width),
+            int(Slices), int(step), int(beginRow), int(beginCol), int(width_red), float(sigma), int(w_num),
+            int(aper_num), float(parameter), _dp(ap_true), mid.ctypes.data_as(_IP), int(ceng_len),
+            *[_dp(a) for a in tail], int(cim_len), _dp(res), _dp(midm) if midm.size else _dp(np.zeros(1)),
+            _dp(px), _dp(py))
+    check(rc)
+    del np2
+    return res, midm, (px if potential_only else None), (py if potential_only else None)
+
+
+def wave_cuda(name, my_real_w, my_img_w, aper, atom_slice, absorp_n2, atom_slice_i, absorp_n_i2, proj_coff_mat,
+              proj_coff_mat_i, Kx, Ky, S_2, my_real_const, my_img_const, gfsf, Height, Width, Depth, P_Height,
+              P_Width, Slices, Step, beginRow, beginCol, width_red, sigma, w_num, aper_num, parameter, aperTrue,
+              absorp_flag, absorp_i_flag, mid_ceng, ceng_len, aper2, seriesn_corr, seriesn_i_corr, elen_corr,
+              elen_i_corr, corrinfo_matrix, cimlen):
+    """The mex function ``waveCuda*_<v>`` with its 41 arguments in ``cuda_stem.m:97`` order
+    (``waveCuda.cpp:18-79``: absorption flags select NULL, cimlen==0 drops the correction table,
+    1-based beginRow/beginCol).  Returns the four mex outputs shaped as MATLAB sees them:
+    myresul [Depth, aper_num], mid_ceng_mat [Depth*aper_num, ceng_len],
+    potentialx/potentialy [P_Height*P_Width, Slices] (zeros for the STEM variants)."""
+    if name not in ENTRY:
+        raise ValueError("unknown mex name %r" % name)
+    res, midm, px, py = my_project_fun(
+        ENTRY[name], my_real_w, my_img_w, aper, atom_slice, absorp_n2 if absorp_flag == 1 else None,
+        atom_slice_i, absorp_n_i2 if absorp_i_flag == 1 else None, proj_coff_mat, proj_coff_mat_i, Kx, Ky, S_2,
+        my_real_const, my_img_const, gfsf, Height, Width, Depth, P_Height, P_Width, Slices, Step, beginRow - 1,
+        beginCol - 1, width_red, sigma, w_num, aper_num, parameter, aperTrue, mid_ceng, ceng_len, aper2,
+        seriesn_corr, seriesn_i_corr, elen_corr, elen_i_corr, corrinfo_matrix if cimlen != 0 else None, cimlen)
+    L = max(int(ceng_len), 0)
+    myresul = res.T.copy()                                   # [Depth, aper_num]
+    mid_mat = midm.reshape(L, aper_num * Depth).T.copy()     # [Depth*aper_num, ceng_len]
+    np2 = P_Height * P_Width
+    if px is None:
+        px = np.zeros((np2, Slices))
+        py = np.zeros((np2, Slices))
+    else:
+        px = px.reshape(Slices, np2).T.copy()
+        py = py.reshape(Slices, np2).T.copy()
+    return myresul, mid_mat, px, py
+
+
+# ------------------------------------------------------------------ device API
+
+
+def build_transmission(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx, ky, s_2, aper_true, sigma,
+                       parameter, t_dev_ptr, absorp_n=None, atom_slice_i=None, proj_coff_mat_i=None,
+                       absorp_n_i=None, series_n_corr=None, ele_n_corr=None, series_n_i_corr=None,
+                       ele_n_i_corr=None, corr_info_matrix=None, want_v=False, stream=None):
+    """Build the transmission stack t (complex64 [S, Np, Np]) into device memory at t_dev_ptr.
+    Returns (V_re, V_im) host arrays when want_v, else None."""
+    keep = {}
+
+    def p(name, a):
+        a = _f64(a)
+        keep[name] = a
+        return _dp(a)
+
+    d = PotentialDesc()
+    d.paraflag = ord(paraflag)
+    d.np = int(np_side)
+    d.slices = int(slices)
+    d.atom_slice = p("as", atom_slice)
+    d.proj_coff_mat = p("pcm", proj_coff_mat)
+    d.absorp_n = p("an", absorp_n)
+    d.atom_slice_i = p("asi", atom_slice_i)
+    d.proj_coff_mat_i = p("pcmi", proj_coff_mat_i)
+    d.absorp_n_i = p("ani", absorp_n_i)
+    d.series_n_corr = p("snc", series_n_corr)
+    d.ele_n_corr = p("enc", ele_n_corr)
+    d.series_n_i_corr = p("snic", series_n_i_corr)
+    d.ele_n_i_corr = p("enic", ele_n_i_corr)
+    d.corr_info_matrix = p("cim", corr_info_matrix)
+    d.cim_len = 0 if corr_info_matrix is None else int(np.asarray(corr_info_matrix).shape[0])
+    d.kx = p("kx", kx)
+    d.ky = p("ky", ky)
+    d.s_2 = p("s2", s_2)
+    d.aper_true = p("at", aper_true)
+    d.sigma = float(sigma)
+    d.parameter = float(parameter)
+    vre = vim = None
+    if want_v:
+        vre = np.zeros((slices, np_side, np_side))
+        vim = np.zeros((slices, np_side, np_side))
+    check(lib().totem_build_transmission(ctypes.byref(d), _VP(t_dev_ptr), _dp(vre), _dp(vim), _VP(stream)))
+    return (vre, vim) if want_v else None
+
+
+def stem_job(n, np_side, slices, t_ptr, probes_ptr, n_probes, prop_ptr, aper_ptr, n_aper, depth, width_red,
+             step, begin_row, begin_col, det_ptr, aper2_ptr=None, mid=(), wrap=False):
+    """Fill a StemJob (device pointers are integers, e.g. torch ``tensor.data_ptr()``)."""
+    j = StemJob()
+    j.n, j.np, j.slices = int(n), int(np_side), int(slices)
+    j.t, j.probes, j.n_probes, j.prop = t_ptr, probes_ptr, int(n_probes), prop_ptr
+    j.aper, j.aper2, j.n_aper = aper_ptr, aper2_ptr, int(n_aper)
+    mid = np.ascontiguousarray(np.asarray(mid, dtype=np.int32))
+    j._mid_keep = mid
+    j.mid = mid.ctypes.data_as(_IP) if mid.size else None
+    j.n_mid = int(mid.size)
+    j.depth, j.width_red, j.step = int(depth), int(width_red), int(step)
+    j.begin_row, j.begin_col, j.wrap = int(begin_row), int(begin_col), int(bool(wrap))
+    j.det = det_ptr
+    return j
+
+
+def stem_workspace_bytes(job, batch):
+    return int(lib().totem_stem_workspace_bytes(ctypes.byref(job), int(batch)))
+
+
+def stem_run(job, batch, graph=False, workspace_ptr=None, workspace_bytes=0, stream=None):
+    check(lib().totem_stem_run(ctypes.byref(job), int(batch), TOTEM_RUN_GRAPH if graph else 0,
+                               _VP(workspace_ptr), ctypes.c_size_t(workspace_bytes), _VP(stream)))
+
+
+def fft2d(ptr, n, batch, sign, stream=None):
+    check(lib().totem_fft2d(_VP(ptr), int(n), int(batch), int(sign), _VP(stream)))
+
+
+def fft_size_supported(n):
+    return bool(lib().totem_fft_size_supported(int(n)))

@@ -1,0 +1,159 @@
+"""STEM simulation driver: configuration -> myProjectfun inputs -> engine.
+
+Mirrors the reference's GUI path ``STEMsimulation`` (``ToTEM_submit.m:1476-1616``)
+and its marshalling script ``cuda_stem.m:1-107``: it builds the specimen rows
+(:mod:`totem_amd.specimen`), the optics (:mod:`totem_amd.optics`) and then the
+41 mex arguments, and calls the engine variant chosen by ``paraflag``
+('l' -> waveCudalobato, 'p'/'n' -> waveCudaPengAndCorr).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import optics, specimen
+
+
+@dataclass
+class StemConfig:
+    atoms: np.ndarray                    # [n, 8] = [Z x y z valence ionicity B occ], x/y inside the potential box
+    np_side: int                         # potential grid side Np (P_Height = P_Width)
+    dx: float                            # sampling, Angstrom/pixel
+    n: int                               # probe window side N (Height = Width)
+    kv: float                            # accelerating voltage
+    alpha_mrad: float                    # probe convergence semi-angle
+    slice_thickness: float               # eachthick, Angstrom
+    n_slices: int
+    slice_origin: float = 0.0            # slicethick[0]
+    paraflag: str = "l"                  # 'l' Lobato, 'p' Peng, 'n' Peng + correction
+    aberrations: optics.Aberrations = None
+    focal_members: int = 1               # mtotal
+    focal_rms_nm: float = 0.0
+    source_mrad: float = 0.0             # spatial-coherence semi-angle (alafa)
+    detectors_mrad: tuple = ((60.0, 200.0),)
+    width_red: int = 1                   # scan columns
+    hight_red: int = 1                   # scan rows
+    step: int = 1                        # scan step, pixels
+    begin_row: int = 1                   # 1-based window origin of the first position (probe_ingreenNrow)
+    begin_col: int = 1
+    mid_slices: tuple = ()               # 1-based mid-slice outputs
+    multiceng: int = 0
+    dw_selected: bool = False            # DW attenuation (radiobutton14/4)
+    tilt_mrad: float = 0.0
+    tilt_phi_deg: float = 0.0
+
+    @property
+    def depth(self):
+        return self.width_red * self.hight_red
+
+
+@dataclass
+class StemInputs:
+    cfg: StemConfig
+    ele_n: np.ndarray
+    ele_n_i: np.ndarray
+    series_n: np.ndarray
+    series_n_i: np.ndarray
+    ele_n_corr: np.ndarray
+    series_n_corr: np.ndarray
+    corr_info: np.ndarray
+    corr_matrix: np.ndarray              # [cim_len, Np, Np] or None
+    gx: np.ndarray
+    gy: np.ndarray
+    s2: np.ndarray
+    aper_true: np.ndarray
+    parameter: float
+    sigma: float
+    probe: np.ndarray                    # [M, N, N] complex128
+    gfsf: np.ndarray
+    psf: np.ndarray                      # [N, N] complex128, ifftshifted
+    aper: np.ndarray                     # [A, N, N] centred
+    aper2: np.ndarray                    # [A, N, N] ifftshifted
+    extra: dict = field(default_factory=dict)
+
+
+def slice_boundaries(cfg):
+    return cfg.slice_origin + cfg.slice_thickness * np.arange(cfg.n_slices)
+
+
+def prepare(cfg, atoms=None):
+    """All engine inputs for one configuration (one frozen-phonon snapshot if ``atoms`` is given)."""
+    x = cfg.atoms if atoms is None else atoms
+    flag = cfg.paraflag
+    st = slice_boundaries(cfg)
+    all_n, all_n_i = specimen.atom_parameters(x, flag)
+    all_n, all_n_i = specimen.fold_debye_waller(all_n, all_n_i, flag, cfg.dw_selected)
+    multiceng = 0 if flag == "l" else cfg.multiceng  # ToTEM_submit.m:874-877
+    dbmode = 1.0 if cfg.dw_selected else 0.0
+    ele_n, ele_n_i, series_n, series_n_i = specimen.slice_atoms(all_n, all_n_i, cfg.slice_thickness, st,
+                                                               multiceng, flag, dbmode)
+    gx, gy, s2 = optics.reciprocal_grids(cfg.np_side, cfg.dx)
+    ele_n_corr = np.zeros((0, 5))
+    series_n_corr = np.zeros(cfg.n_slices)
+    corr_info = np.zeros((0, 20))
+    corr_matrix = None
+    if flag == "n":
+        cn = specimen.corr_parameters(x, cfg.dw_selected)
+        ele_n_corr, corr_info, series_n_corr = specimen.slice_corr(cn, cfg.slice_thickness, st, multiceng)
+        corr_matrix = specimen.corr_info_matrix(corr_info, s2)
+    probe, gfsf = optics.probe_stack(cfg.n, cfg.dx, cfg.kv, cfg.alpha_mrad, cfg.aberrations, cfg.focal_members,
+                                     cfg.focal_rms_nm, cfg.source_mrad, cfg.tilt_mrad, cfg.tilt_phi_deg)
+    psf = optics.propagator(cfg.n, cfg.dx, cfg.kv, cfg.slice_thickness, cfg.tilt_mrad, cfg.tilt_phi_deg)
+    aper = optics.detector_masks(cfg.n, cfg.dx, cfg.kv, cfg.detectors_mrad)
+    aper2 = np.fft.ifftshift(aper, axes=(-2, -1)).copy()
+    if len(series_n) < cfg.n_slices:
+        series_n = np.concatenate([series_n, np.zeros(cfg.n_slices - len(series_n))])
+    return StemInputs(cfg=cfg, ele_n=ele_n, ele_n_i=ele_n_i, series_n=series_n, series_n_i=series_n_i,
+                      ele_n_corr=ele_n_corr, series_n_corr=series_n_corr, corr_info=corr_info,
+                      corr_matrix=corr_matrix, gx=gx, gy=gy, s2=s2, aper_true=np.ones_like(s2),
+                      parameter=optics.potential_parameter(cfg.np_side, cfg.dx),
+                      sigma=optics.interaction_sigma(cfg.kv), probe=probe, gfsf=gfsf, psf=psf, aper=aper,
+                      aper2=aper2)
+
+
+def mex_args(inp):
+    """The 41 mex arguments exactly as ``cuda_stem.m:22-97`` assembles them (row-major numpy views)."""
+    c = inp.cfg
+    S = c.n_slices
+    series_n_i = inp.series_n_i if len(inp.series_n_i) else np.zeros(S)
+    mid = np.asarray(sorted(c.mid_slices), dtype=np.int32)
+    flag = c.paraflag
+    corr = flag == "n" and inp.corr_matrix is not None and len(inp.corr_matrix) > 0
+    return [
+        np.ascontiguousarray(inp.probe.real), np.ascontiguousarray(inp.probe.imag), inp.aper,
+        inp.series_n, None, series_n_i, None,
+        inp.ele_n, inp.ele_n_i if len(inp.ele_n_i) else np.zeros((0, 26 if flag == "l" else 25)),
+        inp.gx, inp.gy, inp.s2,
+        np.ascontiguousarray(inp.psf.real), np.ascontiguousarray(inp.psf.imag), inp.gfsf,
+        c.n, c.n, c.depth, c.np_side, c.np_side, S, c.step, c.begin_row, c.begin_col, c.width_red,
+        inp.sigma, len(inp.gfsf), inp.aper.shape[0], inp.parameter, inp.aper_true,
+        0, 0, mid, len(mid), inp.aper2,
+        inp.series_n_corr if corr else np.zeros(S), np.zeros(S),
+        inp.ele_n_corr if corr else np.zeros((0, 5)), np.zeros((0, 8)),
+        inp.corr_matrix if corr else None, len(inp.corr_matrix) if corr else 0,
+    ]
+
+
+def mex_name(paraflag, potential_only=False):
+    """``cuda_stem.m:82-95``: 'l' -> lobato, 'p'/'n' -> PengAndCorr; 'P' suffix for the potential build."""
+    base = "waveCudalobato" if paraflag == "l" else "waveCudaPengAndCorr"
+    return base + ("P" if potential_only else "")
+
+
+def run(inp, potential_only=False):
+    """Run the engine through the mex-shaped entry point; returns the four mex outputs."""
+    from . import engine
+    return engine.wave_cuda(mex_name(inp.cfg.paraflag, potential_only), *mex_args(inp))
+
+
+# ------------------------------------------------------------------ workloads
+
+def perovskite_c1(n=256, np_side=256, scan=16, step=3):
+    """The small Lobato workload: SrTiO3 5x5x10 cells (1250 atoms), 256^2 over 19.525 A, 20 slices of
+    1.9525 A, 200 kV, 20 mrad, ADF 60-200 mrad, 16x16 scan at 3 px steps (periodic windows)."""
+    a = 3.905
+    x = specimen.perovskite(5, 5, 10, a=a)
+    dz = a / 2
+    return StemConfig(atoms=x, np_side=np_side, dx=5 * a / np_side, n=n, kv=200.0, alpha_mrad=20.0,
+                      slice_thickness=dz, n_slices=20, slice_origin=-dz / 4, paraflag="l",
+                      detectors_mrad=((60.0, 200.0),), width_red=scan, hight_red=scan, step=step,
+                      begin_row=1, begin_col=1)
