@@ -1,0 +1,229 @@
+"""Throughput benchmark of the multislice STEM engine on one MI355X.
+
+Workload (default): frozen-phonon STEM of SrTiO3 20x20x40 cells (80 000 atoms), Peng + correction
+factors, 1024^2 grid over 78.1 A, 80 slices, 300 kV, 24 mrad, Cs 1 um, C1 -5 nm, ADF 70-200 and
+BF 0-10 mrad, 64x64 probe positions (periodic windows), seeded phonon snapshots.  One step is one
+phonon configuration: transmission build on the GPU + all 4096 probes through all 80 slices.
+
+Metric: probe.slice/s (one probe wave through one slice: transmit, 2-D FFT, propagate, inverse FFT),
+with atom rows, probes, propagator and detector masks resident in HBM when the timed region starts.
+
+The JSON line also carries the plain one-probe-per-launch rate (same kernels, batch = 1, no graph),
+the roofline of the dominant kernel from HIP events, and the FP64 oracle timed on the host cores
+on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _cpu_worker_init(path):
+    global _T_HOST
+    _T_HOST = np.load(path, mmap_mode="r")
+
+
+def _cpu_worker(args):
+    from oracle import multislice as oms
+    probe, gfsf, psf, aper_shifted, r0, c0 = args
+    t = np.asarray(_T_HOST, dtype=np.complex128)
+    res, _ = oms.stem_position(t, probe, gfsf, psf, aper_shifted, r0, c0, periodic=True)
+    return res
+
+
+def cpu_baseline(t_dev, inp, positions, cores):
+    """FP64 oracle multislice (numpy pocketfft) over a pool of host processes, one probe per task,
+    like the reference's parfor over scan columns.  Returns (probe.slices/s, wall s, results)."""
+    cfg = inp.cfg
+    t_host = t_dev.cpu().numpy()
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        path = os.path.join(d, "t.npy")
+        np.save(path, t_host)
+        del t_host
+        aper_shifted = np.fft.ifftshift(inp.aper, axes=(-2, -1))
+        tasks = []
+        for dpos in positions:
+            r0 = (cfg.begin_row - 1 + (dpos // cfg.width_red) * cfg.step) % cfg.np_side
+            c0 = (cfg.begin_col - 1 + (dpos % cfg.width_red) * cfg.step) % cfg.np_side
+            tasks.append((inp.probe, inp.gfsf, inp.psf, aper_shifted, r0, c0))
+        ctx = mp.get_context("spawn")
+        with ctx.Pool(cores, initializer=_cpu_worker_init, initargs=(path,)) as pool:
+            pool.map(_cpu_worker, tasks[:1])  # warm the workers (imports, page cache)
+            t0 = time.perf_counter()
+            res = pool.map(_cpu_worker, tasks, chunksize=1)
+            wall = time.perf_counter() - t0
+    return len(positions) * cfg.n_slices / wall, wall, np.asarray(res)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="timed phonon configurations")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
+    ap.add_argument("--graph", type=int, default=0, help="capture the slice loop in a hipGraph")
+    ap.add_argument("--plain-probes", type=int, default=16)
+    ap.add_argument("--cpu-probes", type=int, default=16)
+    ap.add_argument("--cpu-cores", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (multi-GPU sharding is design-only); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from totem_amd import engine, optics, stem
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    engine.lib()
+    dev = torch.device("cuda:0")
+    if args.workload == "c2":
+        cfg = stem.perovskite_c2()
+        sig = stem.C2_PHONON_SIGMA
+        name = ("SrTiO3 20x20x40 cells frozen phonon, Peng+corr, 1024^2, 80 slices, 64x64 probes, "
+                "300 kV 24 mrad Cs 1um C1 -5nm, ADF 70-200 + BF 0-10 mrad")
+    else:
+        cfg = stem.perovskite_c1()
+        sig = {}
+        name = "SrTiO3 5x5x10 cells static, Lobato, 256^2, 20 slices, 16x16 probes, 200 kV 20 mrad, ADF 60-200 mrad"
+    n_cfg = args.warmup + args.steps
+    snaps = stem.phonon_snapshots(cfg, n_cfg, sig, seed=1002) if sig else [cfg.atoms] * n_cfg
+    rows = [stem.prepare_specimen(cfg, x) for x in snaps]
+    inp = stem.prepare(cfg)
+    S, N, NP = cfg.n_slices, cfg.n, cfg.np_side
+    M, A, D = len(inp.gfsf), inp.aper.shape[0], cfg.depth
+
+    probes = torch.tensor(inp.probe.astype(np.complex64), device=dev)
+    prop = torch.tensor(optics.unit_complex64(inp.psf), device=dev)
+    aper = torch.tensor(inp.aper, device=dev)
+    aper2 = torch.tensor(inp.aper2, device=dev)
+    t = torch.empty((S, NP, NP), dtype=torch.complex64, device=dev)
+    det = torch.zeros((1, A, M * D), dtype=torch.float64, device=dev)
+    job = engine.stem_job(N, NP, S, t.data_ptr(), probes.data_ptr(), M, prop.data_ptr(), aper.data_ptr(), A, D,
+                          cfg.width_red, cfg.step, cfg.begin_row - 1, cfg.begin_col - 1, det.data_ptr(),
+                          aper2_ptr=aper2.data_ptr(), wrap=True)
+    batch = M * D
+    ws_bytes = engine.stem_workspace_bytes(job, batch)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def build(r):
+        corr = cfg.paraflag == "n"
+        engine.build_transmission(cfg.paraflag, NP, S, r["series_n"], r["ele_n"], inp.gx, inp.gy, inp.s2,
+                                  inp.aper_true, inp.sigma, inp.parameter, t.data_ptr(),
+                                  series_n_corr=r["series_n_corr"] if corr else None,
+                                  ele_n_corr=r["ele_n_corr"] if corr else None,
+                                  corr_info_matrix=r["corr_matrix"] if corr else None, stream=stream)
+
+    def step(r, timing=False):
+        build(r)
+        engine.stem_run(job, batch, graph=bool(args.graph) and not timing, workspace_ptr=ws.data_ptr(),
+                        workspace_bytes=ws_bytes, stream=stream, timing=timing)
+
+    for k in range(args.warmup):
+        step(rows[k])
+    torch.cuda.synchronize()
+    engine.kernel_times(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(rows[args.warmup + k], timing=not args.graph)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rows_ms, rows_n, cols_ms, cols_n = engine.kernel_times(reset=True)
+    det_last = det.cpu().numpy()[0]
+    probe_slices = args.steps * M * D * S
+    value = probe_slices / wall
+
+    # potential build alone (per phonon configuration)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    build(rows[-1])
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - tb
+    engine.stem_run(job, batch, workspace_ptr=ws.data_ptr(), workspace_bytes=ws_bytes, stream=stream)
+    torch.cuda.synchronize()
+    det_ref = det.cpu().numpy()[0].copy()
+
+    # plain form: same kernels, one probe per launch sequence, no graph
+    pp = min(args.plain_probes, D)
+    det_p = torch.zeros((1, A, M * pp), dtype=torch.float64, device=dev)
+    job_p = engine.stem_job(N, NP, S, t.data_ptr(), probes.data_ptr(), M, prop.data_ptr(), aper.data_ptr(), A, pp,
+                            cfg.width_red, cfg.step, cfg.begin_row - 1, cfg.begin_col - 1, det_p.data_ptr(),
+                            aper2_ptr=aper2.data_ptr(), wrap=True)
+    engine.stem_run(job_p, 1, workspace_ptr=ws.data_ptr(), workspace_bytes=ws_bytes, stream=stream)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    engine.stem_run(job_p, 1, workspace_ptr=ws.data_ptr(), workspace_bytes=ws_bytes, stream=stream)
+    torch.cuda.synchronize()
+    plain_wall = time.perf_counter() - tp
+    plain_value = M * pp * S / plain_wall
+    plain_match = bool(np.array_equal(det_p.cpu().numpy()[0], det_ref[:, :M * pp]))
+
+    # roofline of the dominant kernel (HIP events on the launch stream)
+    n2 = float(N) * N
+    bytes_rows = batch * 16.0 * n2 + 8.0 * NP * NP          # psi read+write + the shared t slice
+    bytes_cols = batch * 16.0 * n2 + 8.0 * n2                # psi read+write + the propagator
+    rows_avg = rows_ms / max(rows_n, 1)
+    cols_avg = cols_ms / max(cols_n, 1)
+    if rows_avg >= cols_avg:
+        kname, kavg, kbytes = "k_pass_rows<%d>" % N, rows_avg, bytes_rows
+    else:
+        kname, kavg, kbytes = "k_pass_cols<%d>" % N, cols_avg, bytes_cols
+    achieved = kbytes / (kavg * 1e-3) / 1e9 if kavg > 0 else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(kname)
+
+    out = {
+        "metric": "probe.slice/s at 1024^2 FP32 (batched vs plain one-probe-per-launch form)",
+        "value": value,
+        "unit": "probe.slice/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded specimen and phonon snapshots)",
+        "config": {"workload": name, "grid": N, "slices": S, "probes_per_step": M * D,
+                   "phonon_configs_timed": args.steps, "paraflag": cfg.paraflag, "graph": bool(args.graph)},
+        "plain": {"value": plain_value, "probes": pp, "speedup_batched_vs_plain": value / plain_value,
+                  "bit_identical_to_batched": plain_match},
+        "potential_build_s_per_config": build_s,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": kavg,
+                     "rows_avg_ms": rows_avg, "cols_avg_ms": cols_avg},
+    }
+    if not args.no_cpu and args.cpu_probes > 0:
+        cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
+        positions = list(range(min(args.cpu_probes, D)))
+        cv, cwall, cres = cpu_baseline(t, inp, positions, cores)
+        gpu = det_ref.T[positions]
+        rel = float(np.max(np.abs(gpu - cres) / np.abs(cres)))
+        out["cpu_baseline"] = {"value": cv, "unit": "probe.slice/s", "cores": cores, "kind": "port",
+                               "sample": "%d probe positions x %d slices of phonon configuration %d, FP64 numpy "
+                                         "oracle (propagation only, transmission taken from the GPU build), "
+                                         "%.1f s wall" % (len(positions), S, n_cfg - 1, cwall)}
+        out["oracle_max_rel_err"] = rel
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

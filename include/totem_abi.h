@@ -133,6 +133,7 @@ typedef struct totem_stem_job {
 } totem_stem_job;
 
 #define TOTEM_RUN_GRAPH 1     /* capture the slice loop of a batch in a hipGraph and replay it */
+#define TOTEM_RUN_TIMING 2    /* time every rows/columns pass with HIP events (no graph; synchronises) */
 
 /* Device bytes of workspace totem_stem_run needs for `batch` probes in flight. */
 size_t totem_stem_workspace_bytes(const totem_stem_job *job, int batch);
@@ -141,6 +142,10 @@ size_t totem_stem_workspace_bytes(const totem_stem_job *job, int batch);
  * Asynchronous on `stream` (NULL: the library's stream) unless it allocates. */
 int totem_stem_run(const totem_stem_job *job, int batch, int flags, void *workspace, size_t ws_bytes,
                    void *stream);
+
+/* Accumulated durations (ms) and launch counts of the rows and columns passes
+ * over TOTEM_RUN_TIMING runs on the calling thread; reset != 0 clears them after reading. */
+void totem_kernel_times(int reset, double *rows_ms, long *rows_n, double *cols_ms, long *cols_n);
 
 /* Batched in-place 2-D FFT of `batch` n x n complex64 matrices on the device
  * (sign -1 forward, +1 inverse, unnormalised).  Test/diagnostic entry point. */

@@ -23,10 +23,20 @@ specimen; focal-series linearity; scan/translation equivalence.
 import numpy as np
 
 
-def stem_position(t, probe, gfsf, psf_fft, aper_shifted, r0, c0, midslice=()):
-    """One scan position, all focal members: (final [A], mid [L, A]) in FP64."""
+def stem_position(t, probe, gfsf, psf_fft, aper_shifted, r0, c0, midslice=(), periodic=False):
+    """One scan position, all focal members: (final [A], mid [L, A]) in FP64.
+
+    ``periodic`` takes the window modulo the potential side (the periodic extension
+    of the reciprocal-space potential); the reference crops without wrapping."""
     S = t.shape[0]
     n = probe.shape[-1]
+    if periodic:
+        npx = t.shape[-1]
+        ri = (r0 + np.arange(n)) % npx
+        ci = (c0 + np.arange(n)) % npx
+        window = lambda k: t[k - 1][np.ix_(ri, ci)]
+    else:
+        window = lambda k: t[k - 1, r0:r0 + n, c0:c0 + n]
     A = aper_shifted.shape[0]
     mids = list(midslice)
     res = np.zeros(A)
@@ -34,7 +44,7 @@ def stem_position(t, probe, gfsf, psf_fft, aper_shifted, r0, c0, midslice=()):
     for nn in range(len(gfsf)):
         wave = probe[nn].astype(np.complex128)
         for k in range(1, S + 1):
-            wave = wave * t[k - 1, r0:r0 + n, c0:c0 + n]
+            wave = wave * window(k)
             wave = np.fft.fft2(wave)
             wave = wave * psf_fft
             if k in mids:

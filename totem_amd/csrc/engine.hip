@@ -111,6 +111,32 @@ static Context& ctx() {
   return *c;
 }
 
+// FP32 pair for a unit-modulus FP64 value: among the neighbouring floats of each
+// component, the pair whose squared modulus is closest to 1 (ties: closest value).
+// Rounding the components independently leaves |w|^2 - 1 of a few 1e-8 with a
+// fixed sign per entry; the same twiddles and propagator entries are applied
+// hundreds of times per probe, so that bias would accumulate coherently.
+static float2 unit_float2(double c, double s) {
+  const float c0 = (float)c, s0 = (float)s;
+  float best_c = c0, best_s = s0;
+  double best = 1e300;
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j) {
+      float cc = c0, ss = s0;
+      if (i) cc = std::nextafter(c0, i < 0 ? -2.0f : 2.0f);
+      if (j) ss = std::nextafter(s0, j < 0 ? -2.0f : 2.0f);
+      const double dn = std::fabs((double)cc * cc + (double)ss * ss - 1.0);
+      const double dv = std::fabs(cc - c) + std::fabs(ss - s);
+      const double score = dn * 1e6 + dv;
+      if (score < best) {
+        best = score;
+        best_c = cc;
+        best_s = ss;
+      }
+    }
+  return make_float2(best_c, best_s);
+}
+
 const float2* tw_table(int n) {
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
@@ -122,7 +148,7 @@ const float2* tw_table(int n) {
   std::vector<float2> h(n);
   for (int m = 0; m < n; ++m) {
     const double a = -2.0 * M_PI * (double)m / (double)n;
-    h[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    h[m] = unit_float2(std::cos(a), std::sin(a));
   }
   void* p = nullptr;
   HIP_OK(hipMalloc(&p, n * sizeof(float2)));
@@ -466,6 +492,14 @@ static void validate_job(const totem_stem_job& j) {
   }
 }
 
+// Per-kernel durations of the last timed run (TOTEM_RUN_TIMING), from HIP events
+// recorded on the launch stream around every rows/columns pass.
+struct KernelTimes {
+  double rows_ms = 0.0, cols_ms = 0.0;
+  long rows_n = 0, cols_n = 0;
+};
+static thread_local KernelTimes g_times;
+
 static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* ws, size_t ws_bytes, hipStream_t st) {
   validate_job(j);
   const int n = j.n, S = j.slices, A = j.n_aper, L = j.n_mid, M = j.n_probes, D = j.depth;
@@ -543,7 +577,29 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
       }
       reduce_det(st, w.items, nullptr, cnt, w.job, w.partial, G, L + 1, A);
     };
-    if (flags & TOTEM_RUN_GRAPH) {
+    if (flags & TOTEM_RUN_TIMING) {
+      std::vector<hipEvent_t> ev(2 * S + 1);
+      for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+      HIP_OK(hipEventRecord(ev[0], st));
+      for (int s = 0; s < S; ++s) {
+        pass_rows(n, st, w.items, cnt, w.job, w.psi, s);
+        HIP_OK(hipEventRecord(ev[2 * s + 1], st));
+        pass_cols(n, st, w.items, cnt, w.job, w.psi, s, w.partial, L + 1, A);
+        HIP_OK(hipEventRecord(ev[2 * s + 2], st));
+      }
+      reduce_det(st, w.items, nullptr, cnt, w.job, w.partial, G, L + 1, A);
+      HIP_OK(hipEventSynchronize(ev[2 * S]));
+      for (int s = 0; s < S; ++s) {
+        float a = 0.f, b = 0.f;
+        HIP_OK(hipEventElapsedTime(&a, ev[2 * s], ev[2 * s + 1]));
+        HIP_OK(hipEventElapsedTime(&b, ev[2 * s + 1], ev[2 * s + 2]));
+        g_times.rows_ms += a;
+        g_times.cols_ms += b;
+        g_times.rows_n += 1;
+        g_times.cols_n += 1;
+      }
+      for (auto& e : ev) HIP_OK(hipEventDestroy(e));
+    } else if (flags & TOTEM_RUN_GRAPH) {
       if (graph_count != cnt) {
         if (gexec) HIP_OK(hipGraphExecDestroy(gexec));
         hipGraph_t g;
@@ -682,7 +738,10 @@ static void project_impl(const ProjectArgs& a, int paraflag, bool potential_only
   // probes and propagator: double -> complex64 (initKernel_W, cuda_source_lobato.cu:39-57; :1207-1220)
   std::vector<float2> hp(n2 * M), hh(n2);
   for (size_t i = 0; i < n2 * M; ++i) hp[i] = make_float2((float)a.my_real_w[i], (float)a.my_imag_w[i]);
-  for (size_t i = 0; i < n2; ++i) hh[i] = make_float2((float)a.my_real_const[i], (float)a.my_img_const[i]);
+  for (size_t i = 0; i < n2; ++i) {
+    const double re = a.my_real_const[i], im = a.my_img_const[i];
+    hh[i] = std::fabs(re * re + im * im - 1.0) < 1e-12 ? unit_float2(re, im) : make_float2((float)re, (float)im);
+  }
   DevBuf dprobe, dprop, daper, daper2, ddet;
   upload(dprobe, hp, st);
   upload(dprop, hh, st);
@@ -784,6 +843,14 @@ int totem_stem_run(const totem_stem_job* job, int batch, int flags, void* worksp
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : lib_stream();
     run_stem_impl(*job, batch, flags, workspace, ws_bytes, st);
   });
+}
+
+void totem_kernel_times(int reset, double* rows_ms, long* rows_n, double* cols_ms, long* cols_n) {
+  if (rows_ms) *rows_ms = g_times.rows_ms;
+  if (rows_n) *rows_n = g_times.rows_n;
+  if (cols_ms) *cols_ms = g_times.cols_ms;
+  if (cols_n) *cols_n = g_times.cols_n;
+  if (reset) g_times = KernelTimes();
 }
 
 int totem_fft2d(void* data, int n, int batch, int sign, void* stream) {

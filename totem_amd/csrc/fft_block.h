@@ -25,10 +25,33 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 template <int SIGN> __device__ __forceinline__ float2 mul_si(float2 a) {
   return SIGN < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
 }
-// a * (c + SIGN*i*s)
-template <int SIGN> __device__ __forceinline__ float2 mul_cs(float2 a, float c, float s) {
-  const float ss = SIGN < 0 ? -s : s;
-  return make_float2(a.x * c - a.y * ss, a.x * ss + a.y * c);
+// Butterfly constants as (hi, lo) float pairs: hi is the nearest float, lo the
+// float nearest the remainder.  Multiplying by hi alone scales |w|^2 by 1 - O(1e-8)
+// with the same sign every time (e.g. 2 * float(sqrt(1/2))^2 = 1 - 3.4e-8); the
+// multislice loop applies each butterfly hundreds of times to similar data, so
+// that bias would add up coherently (a 1.8e-5 intensity loss over 80 slices at
+// 1024^2).  With the lo term folded in by FMA the products round once, unbiased.
+struct K2 {
+  float h, l;
+};
+constexpr K2 kR = {7.071067691e-01f, 1.210161749e-08f};     // sqrt(1/2)
+constexpr K2 kC16 = {9.238795042e-01f, 2.830748969e-08f};   // cos(pi/8)
+constexpr K2 kS16 = {3.826834261e-01f, 6.223350724e-09f};   // sin(pi/8)
+constexpr K2 kS3 = {8.660253882e-01f, 1.554362505e-08f};    // sqrt(3)/2
+constexpr K2 kC51 = {3.090170026e-01f, -8.207602598e-09f};  // cos(2 pi/5)
+constexpr K2 kC52 = {-8.090170026e-01f, 8.207602598e-09f};  // cos(4 pi/5)
+constexpr K2 kS51 = {9.510565400e-01f, -2.371720598e-08f};  // sin(2 pi/5)
+constexpr K2 kS52 = {5.877852440e-01f, 8.304436250e-09f};   // sin(4 pi/5)
+
+__device__ __forceinline__ float kmul(float v, K2 k) { return fmaf(v, k.h, v * k.l); }
+__device__ __forceinline__ K2 kneg(K2 k) { return K2{-k.h, -k.l}; }
+
+// a * (c + SIGN*i*s) with (hi, lo) constants
+template <int SIGN> __device__ __forceinline__ float2 mul_cs(float2 a, K2 c, K2 s) {
+  const K2 ss = SIGN < 0 ? kneg(s) : s;
+  const float re = fmaf(a.x, c.h, fmaf(-a.y, ss.h, fmaf(a.x, c.l, -a.y * ss.l)));
+  const float im = fmaf(a.x, ss.h, fmaf(a.y, c.h, fmaf(a.x, ss.l, a.y * c.l)));
+  return make_float2(re, im);
 }
 
 // ------------------------------------------------------------------ small DFTs
@@ -41,12 +64,11 @@ template <int SIGN> __device__ __forceinline__ void dft2(float2* a) {
 }
 
 template <int SIGN> __device__ __forceinline__ void dft3(float2* a) {
-  const float s3 = 0.86602540378443865f;
   float2 t1 = cadd(a[1], a[2]);
   float2 t2 = csub(a[1], a[2]);
   float2 y0 = cadd(a[0], t1);
   float2 m = make_float2(a[0].x - 0.5f * t1.x, a[0].y - 0.5f * t1.y);
-  float2 n = mul_si<SIGN>(cscale(t2, s3));
+  float2 n = mul_si<SIGN>(make_float2(kmul(t2.x, kS3), kmul(t2.y, kS3)));
   a[0] = y0;
   a[1] = cadd(m, n);
   a[2] = csub(m, n);
@@ -63,16 +85,19 @@ template <int SIGN> __device__ __forceinline__ void dft4(float2* a) {
   a[3] = csub(t1, t3);
 }
 
+// c1*u + c2*v with (hi, lo) constants, one rounding per FMA
+__device__ __forceinline__ float kdot(float u, K2 c1, float v, K2 c2, float base) {
+  return fmaf(u, c1.h, fmaf(v, c2.h, base + (u * c1.l + v * c2.l)));
+}
+
 template <int SIGN> __device__ __forceinline__ void dft5(float2* a) {
-  const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
-  const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
   float2 t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
   float2 t3 = csub(a[1], a[4]), t4 = csub(a[2], a[3]);
   float2 y0 = cadd(a[0], cadd(t1, t2));
-  float2 b1 = make_float2(a[0].x + c1 * t1.x + c2 * t2.x, a[0].y + c1 * t1.y + c2 * t2.y);
-  float2 b2 = make_float2(a[0].x + c2 * t1.x + c1 * t2.x, a[0].y + c2 * t1.y + c1 * t2.y);
-  float2 d1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
-  float2 d2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+  float2 b1 = make_float2(kdot(t1.x, kC51, t2.x, kC52, a[0].x), kdot(t1.y, kC51, t2.y, kC52, a[0].y));
+  float2 b2 = make_float2(kdot(t1.x, kC52, t2.x, kC51, a[0].x), kdot(t1.y, kC52, t2.y, kC51, a[0].y));
+  float2 d1 = make_float2(kdot(t3.x, kS51, t4.x, kS52, 0.f), kdot(t3.y, kS51, t4.y, kS52, 0.f));
+  float2 d2 = make_float2(kdot(t3.x, kS52, t4.x, kneg(kS51), 0.f), kdot(t3.y, kS52, t4.y, kneg(kS51), 0.f));
   float2 e1 = mul_si<SIGN>(d1), e2 = mul_si<SIGN>(d2);
   a[0] = y0;
   a[1] = cadd(b1, e1);
@@ -81,15 +106,26 @@ template <int SIGN> __device__ __forceinline__ void dft5(float2* a) {
   a[3] = csub(b2, e2);
 }
 
+// a * (1 + SIGN*i)/sqrt2 and a * (-1 + SIGN*i)/sqrt2
+template <int SIGN> __device__ __forceinline__ float2 mul_w8(float2 a) {
+  const float u = SIGN < 0 ? a.x + a.y : a.x - a.y;
+  const float v = SIGN < 0 ? a.y - a.x : a.x + a.y;
+  return make_float2(kmul(u, kR), kmul(v, kR));
+}
+template <int SIGN> __device__ __forceinline__ float2 mul_w83(float2 a) {
+  const float u = SIGN < 0 ? a.y - a.x : -a.x - a.y;
+  const float v = SIGN < 0 ? -a.x - a.y : a.x - a.y;
+  return make_float2(kmul(u, kR), kmul(v, kR));
+}
+
 template <int SIGN> __device__ __forceinline__ void dft8(float2* a) {
-  const float r = 0.70710678118654752f;
   float2 e[4] = {a[0], a[2], a[4], a[6]};
   float2 o[4] = {a[1], a[3], a[5], a[7]};
   dft4<SIGN>(e);
   dft4<SIGN>(o);
-  o[1] = mul_cs<SIGN>(o[1], r, r);
+  o[1] = mul_w8<SIGN>(o[1]);
   o[2] = mul_si<SIGN>(o[2]);
-  o[3] = mul_cs<SIGN>(o[3], -r, r);
+  o[3] = mul_w83<SIGN>(o[3]);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     a[k] = cadd(e[k], o[k]);
@@ -98,7 +134,6 @@ template <int SIGN> __device__ __forceinline__ void dft8(float2* a) {
 }
 
 template <int SIGN> __device__ __forceinline__ void dft16(float2* a) {
-  const float c1 = 0.92387953251128674f, s1 = 0.38268343236508978f, r = 0.70710678118654752f;
   float2 e[8], o[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -107,13 +142,13 @@ template <int SIGN> __device__ __forceinline__ void dft16(float2* a) {
   }
   dft8<SIGN>(e);
   dft8<SIGN>(o);
-  o[1] = mul_cs<SIGN>(o[1], c1, s1);
-  o[2] = mul_cs<SIGN>(o[2], r, r);
-  o[3] = mul_cs<SIGN>(o[3], s1, c1);
+  o[1] = mul_cs<SIGN>(o[1], kC16, kS16);
+  o[2] = mul_w8<SIGN>(o[2]);
+  o[3] = mul_cs<SIGN>(o[3], kS16, kC16);
   o[4] = mul_si<SIGN>(o[4]);
-  o[5] = mul_cs<SIGN>(o[5], -s1, c1);
-  o[6] = mul_cs<SIGN>(o[6], -r, r);
-  o[7] = mul_cs<SIGN>(o[7], -c1, s1);
+  o[5] = mul_cs<SIGN>(o[5], kneg(kS16), kC16);
+  o[6] = mul_w83<SIGN>(o[6]);
+  o[7] = mul_cs<SIGN>(o[7], kneg(kC16), kS16);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     a[k] = cadd(e[k], o[k]);

@@ -34,10 +34,11 @@ EXPORTS = (
     "myProjectfun",
     "totem_project_lobato", "totem_project_peng", "totem_potential_lobato", "totem_potential_peng",
     "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
-    "totem_stem_workspace_bytes", "totem_stem_run", "totem_fft2d",
+    "totem_stem_workspace_bytes", "totem_stem_run", "totem_kernel_times", "totem_fft2d",
 )
 
 TOTEM_RUN_GRAPH = 1
+TOTEM_RUN_TIMING = 2
 
 
 class TotemError(RuntimeError):
@@ -108,6 +109,8 @@ def lib():
         L.totem_stem_workspace_bytes.restype = ctypes.c_size_t
         L.totem_stem_run.argtypes = [ctypes.POINTER(StemJob), _I, _I, _VP, ctypes.c_size_t, _VP]
         L.totem_stem_run.restype = _I
+        L.totem_kernel_times.argtypes = [_I, _DP, ctypes.POINTER(ctypes.c_long), _DP, ctypes.POINTER(ctypes.c_long)]
+        L.totem_kernel_times.restype = None
         L.totem_fft2d.argtypes = [_VP, _I, _I, _I, _VP]
         L.totem_fft2d.restype = _I
         _lib = L
@@ -262,9 +265,18 @@ def stem_workspace_bytes(job, batch):
     return int(lib().totem_stem_workspace_bytes(ctypes.byref(job), int(batch)))
 
 
-def stem_run(job, batch, graph=False, workspace_ptr=None, workspace_bytes=0, stream=None):
-    check(lib().totem_stem_run(ctypes.byref(job), int(batch), TOTEM_RUN_GRAPH if graph else 0,
-                               _VP(workspace_ptr), ctypes.c_size_t(workspace_bytes), _VP(stream)))
+def stem_run(job, batch, graph=False, workspace_ptr=None, workspace_bytes=0, stream=None, timing=False):
+    flags = (TOTEM_RUN_GRAPH if graph else 0) | (TOTEM_RUN_TIMING if timing else 0)
+    check(lib().totem_stem_run(ctypes.byref(job), int(batch), flags, _VP(workspace_ptr),
+                               ctypes.c_size_t(workspace_bytes), _VP(stream)))
+
+
+def kernel_times(reset=True):
+    """(rows_ms, rows_launches, cols_ms, cols_launches) accumulated by timed runs."""
+    rm, cm = ctypes.c_double(), ctypes.c_double()
+    rn, cn = ctypes.c_long(), ctypes.c_long()
+    lib().totem_kernel_times(int(reset), ctypes.byref(rm), ctypes.byref(rn), ctypes.byref(cm), ctypes.byref(cn))
+    return rm.value, rn.value, cm.value, cn.value
 
 
 def fft2d(ptr, n, batch, sign, stream=None):

@@ -254,3 +254,34 @@ def probe_stack(n, dx, kv, alpha_mrad, ab=None, mtotal=1, focal_rms_nm=0.0, sour
         norm = np.sqrt(np.sum(np.abs(w) ** 2))
         probes.append(np.fft.fftshift(w) / norm)
     return np.stack(probes), np.asarray(gfsf, dtype=np.float64)
+
+
+def unit_complex64(h):
+    """complex128 -> complex64 keeping unit-modulus entries as close to |h| = 1 as FP32 allows.
+
+    Each component is taken from its nearest float or one of that float's two
+    neighbours, choosing the pair whose squared modulus is nearest 1 (ties: the
+    closer value); entries that are not unit-modulus (e.g. the zeros outside the
+    band limit) are rounded normally.  The engine's C ABI applies the same rule to
+    the propagator it receives in double precision.
+    """
+    h = np.asarray(h, dtype=np.complex128)
+    c, s = h.real, h.imag
+    c0, s0 = c.astype(np.float32), s.astype(np.float32)
+    best_c, best_s = c0.copy(), s0.copy()
+    best = np.full(h.shape, np.inf)
+    for i in (-1, 0, 1):
+        cc = c0 if i == 0 else np.nextafter(c0, np.float32(2.0 * i))
+        for j in (-1, 0, 1):
+            ss = s0 if j == 0 else np.nextafter(s0, np.float32(2.0 * j))
+            dn = np.abs(cc.astype(np.float64) ** 2 + ss.astype(np.float64) ** 2 - 1.0)
+            dv = np.abs(cc - c) + np.abs(ss - s)
+            score = dn * 1e6 + dv
+            better = score < best
+            best = np.where(better, score, best)
+            best_c = np.where(better, cc, best_c)
+            best_s = np.where(better, ss, best_s)
+    unit = np.abs(np.abs(h) ** 2 - 1.0) < 1e-12
+    out = h.astype(np.complex64)
+    out[unit] = (best_c + 1j * best_s).astype(np.complex64)[unit]
+    return out

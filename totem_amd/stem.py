@@ -75,7 +75,31 @@ def slice_boundaries(cfg):
     return cfg.slice_origin + cfg.slice_thickness * np.arange(cfg.n_slices)
 
 
-def prepare(cfg, atoms=None):
+def prepare_specimen(cfg, atoms=None):
+    """Per-slice coefficient rows of one specimen snapshot: dict with ele_n, ele_n_i, series_n,
+    series_n_i, ele_n_corr, series_n_corr, corr_info, corr_matrix."""
+    inp = prepare(cfg, atoms, optics_too=False)
+    return {k: getattr(inp, k) for k in ("ele_n", "ele_n_i", "series_n", "series_n_i", "ele_n_corr",
+                                         "series_n_corr", "corr_info", "corr_matrix")}
+
+
+def phonon_snapshots(cfg, count, sigma_by_z, seed=0):
+    """Frozen-phonon atom matrices: every atom displaced by an isotropic Gaussian of rms sigma_by_z[Z]
+    (B = 8 pi^2 sigma^2, ``randshiftpos.m:4``), redrawn to stay inside the slab.  Seeded."""
+    x = cfg.atoms.copy()
+    for z, sg in sigma_by_z.items():
+        x[x[:, 0] == z, 6] = 8 * np.pi ** 2 * sg ** 2
+    st = slice_boundaries(cfg)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        xs = specimen.frozen_phonon(x, cfg.slice_thickness, st, tuple(cfg.mid_slices), rng, bcol=6)
+        xs[:, 6] = 0.0  # frozen phonon only: no Debye-Waller attenuation on top of the displacements
+        out.append(xs)
+    return out
+
+
+def prepare(cfg, atoms=None, optics_too=True):
     """All engine inputs for one configuration (one frozen-phonon snapshot if ``atoms`` is given)."""
     x = cfg.atoms if atoms is None else atoms
     flag = cfg.paraflag
@@ -95,11 +119,14 @@ def prepare(cfg, atoms=None):
         cn = specimen.corr_parameters(x, cfg.dw_selected)
         ele_n_corr, corr_info, series_n_corr = specimen.slice_corr(cn, cfg.slice_thickness, st, multiceng)
         corr_matrix = specimen.corr_info_matrix(corr_info, s2)
-    probe, gfsf = optics.probe_stack(cfg.n, cfg.dx, cfg.kv, cfg.alpha_mrad, cfg.aberrations, cfg.focal_members,
-                                     cfg.focal_rms_nm, cfg.source_mrad, cfg.tilt_mrad, cfg.tilt_phi_deg)
-    psf = optics.propagator(cfg.n, cfg.dx, cfg.kv, cfg.slice_thickness, cfg.tilt_mrad, cfg.tilt_phi_deg)
-    aper = optics.detector_masks(cfg.n, cfg.dx, cfg.kv, cfg.detectors_mrad)
-    aper2 = np.fft.ifftshift(aper, axes=(-2, -1)).copy()
+    if optics_too:
+        probe, gfsf = optics.probe_stack(cfg.n, cfg.dx, cfg.kv, cfg.alpha_mrad, cfg.aberrations, cfg.focal_members,
+                                         cfg.focal_rms_nm, cfg.source_mrad, cfg.tilt_mrad, cfg.tilt_phi_deg)
+        psf = optics.propagator(cfg.n, cfg.dx, cfg.kv, cfg.slice_thickness, cfg.tilt_mrad, cfg.tilt_phi_deg)
+        aper = optics.detector_masks(cfg.n, cfg.dx, cfg.kv, cfg.detectors_mrad)
+        aper2 = np.fft.ifftshift(aper, axes=(-2, -1)).copy()
+    else:
+        probe = gfsf = psf = aper = aper2 = None
     if len(series_n) < cfg.n_slices:
         series_n = np.concatenate([series_n, np.zeros(cfg.n_slices - len(series_n))])
     return StemInputs(cfg=cfg, ele_n=ele_n, ele_n_i=ele_n_i, series_n=series_n, series_n_i=series_n_i,
@@ -157,3 +184,20 @@ def perovskite_c1(n=256, np_side=256, scan=16, step=3):
                       slice_thickness=dz, n_slices=20, slice_origin=-dz / 4, paraflag="l",
                       detectors_mrad=((60.0, 200.0),), width_red=scan, hight_red=scan, step=step,
                       begin_row=1, begin_col=1)
+
+
+def perovskite_c2(scan=64, step=3):
+    """The frozen-phonon workload: SrTiO3 20x20x40 cells (80 000 atoms), 1024^2 over 78.1 A, 80 slices of
+    1.9525 A, Peng + correction, 300 kV, 24 mrad, Cs = 1 um, C1 = -5 nm, ADF 70-200 and BF 0-10 mrad,
+    64x64 scan at 3 px (periodic windows).  Phonon rms 0.08 A (Sr, Ti), 0.10 A (O)."""
+    a = 3.905
+    x = specimen.perovskite(20, 20, 40, a=a)
+    dz = a / 2
+    return StemConfig(atoms=x, np_side=1024, dx=20 * a / 1024, n=1024, kv=300.0, alpha_mrad=24.0,
+                      slice_thickness=dz, n_slices=80, slice_origin=-dz / 4, paraflag="n",
+                      aberrations=optics.Aberrations(C1=-5.0, Cs_um=1.0),
+                      detectors_mrad=((70.0, 200.0), (0.0, 10.0)), width_red=scan, hight_red=scan, step=step,
+                      begin_row=1, begin_col=1)
+
+
+C2_PHONON_SIGMA = {38: 0.08, 22: 0.08, 8: 0.10}
