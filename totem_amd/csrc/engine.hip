@@ -33,6 +33,7 @@
 #include "../../include/totem_abi.h"
 #include "kernels.h"
 #include "launch.h"
+#include "unit_round.h"
 
 namespace totem {
 
@@ -111,32 +112,6 @@ static Context& ctx() {
   return *c;
 }
 
-// FP32 pair for a unit-modulus FP64 value: among the neighbouring floats of each
-// component, the pair whose squared modulus is closest to 1 (ties: closest value).
-// Rounding the components independently leaves |w|^2 - 1 of a few 1e-8 with a
-// fixed sign per entry; the same twiddles and propagator entries are applied
-// hundreds of times per probe, so that bias would accumulate coherently.
-static float2 unit_float2(double c, double s) {
-  const float c0 = (float)c, s0 = (float)s;
-  float best_c = c0, best_s = s0;
-  double best = 1e300;
-  for (int i = -1; i <= 1; ++i)
-    for (int j = -1; j <= 1; ++j) {
-      float cc = c0, ss = s0;
-      if (i) cc = std::nextafter(c0, i < 0 ? -2.0f : 2.0f);
-      if (j) ss = std::nextafter(s0, j < 0 ? -2.0f : 2.0f);
-      const double dn = std::fabs((double)cc * cc + (double)ss * ss - 1.0);
-      const double dv = std::fabs(cc - c) + std::fabs(ss - s);
-      const double score = dn * 1e6 + dv;
-      if (score < best) {
-        best = score;
-        best_c = cc;
-        best_s = ss;
-      }
-    }
-  return make_float2(best_c, best_s);
-}
-
 const float2* tw_table(int n) {
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
@@ -145,14 +120,12 @@ const float2* tw_table(int n) {
   auto key = std::make_pair(dev, n);
   auto it = c.tw.find(key);
   if (it != c.tw.end()) return static_cast<const float2*>(it->second);
-  std::vector<float2> h(n);
-  for (int m = 0; m < n; ++m) {
-    const double a = -2.0 * M_PI * (double)m / (double)n;
-    h[m] = unit_float2(std::cos(a), std::sin(a));
-  }
+  std::vector<float2> h;
+  twiddle_host(n, h);
+  require(!h.empty() || n <= 16, "no twiddle table for n=" + std::to_string(n), TOTEM_ERR_UNSUPPORTED);
   void* p = nullptr;
-  HIP_OK(hipMalloc(&p, n * sizeof(float2)));
-  HIP_OK(hipMemcpy(p, h.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+  HIP_OK(hipMalloc(&p, std::max<size_t>(h.size(), 1) * sizeof(float2)));
+  if (!h.empty()) HIP_OK(hipMemcpy(p, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
   c.tw[key] = p;
   return static_cast<const float2*>(p);
 }
@@ -573,7 +546,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
     auto record = [&]() {
       for (int s = 0; s < S; ++s) {
         pass_rows(n, st, w.items, cnt, w.job, w.psi, s);
-        pass_cols(n, st, w.items, cnt, w.job, w.psi, s, w.partial, L + 1, A);
+        pass_cols(n, st, w.items, cnt, w.job, w.psi, s, w.partial, L + 1, A, s == S - 1 || mos[s] >= 0);
       }
       reduce_det(st, w.items, nullptr, cnt, w.job, w.partial, G, L + 1, A);
     };
@@ -584,7 +557,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
       for (int s = 0; s < S; ++s) {
         pass_rows(n, st, w.items, cnt, w.job, w.psi, s);
         HIP_OK(hipEventRecord(ev[2 * s + 1], st));
-        pass_cols(n, st, w.items, cnt, w.job, w.psi, s, w.partial, L + 1, A);
+        pass_cols(n, st, w.items, cnt, w.job, w.psi, s, w.partial, L + 1, A, s == S - 1 || mos[s] >= 0);
         HIP_OK(hipEventRecord(ev[2 * s + 2], st));
       }
       reduce_det(st, w.items, nullptr, cnt, w.job, w.partial, G, L + 1, A);

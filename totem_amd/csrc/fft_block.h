@@ -15,6 +15,38 @@
 
 namespace totem {
 
+// Global-memory accessors.  Pointers read from device-side structs are generic to
+// the compiler; flat loads count against both the vector-memory and the LDS
+// counters, so a wait for one serialises the other.  These assert the global
+// address space (8-byte accesses; the float2 is reassembled in registers).
+typedef __attribute__((address_space(1))) const unsigned long long* g64c_t;
+typedef __attribute__((address_space(1))) unsigned long long* g64_t;
+__device__ __forceinline__ float2 gld(const float2* p) {
+  const unsigned long long v = *(g64c_t)(p);
+  return make_float2(__uint_as_float((unsigned)(v & 0xffffffffull)), __uint_as_float((unsigned)(v >> 32)));
+}
+__device__ __forceinline__ void gst(float2* p, float2 v) {
+  *(g64_t)(p) = ((unsigned long long)__float_as_uint(v.y) << 32) | (unsigned long long)__float_as_uint(v.x);
+}
+__device__ __forceinline__ double gldd(const double* p) { return *(__attribute__((address_space(1))) const double*)(p); }
+
+// Raw buffer access: a 128-bit resource (base, byte range) in SGPRs plus a 32-bit
+// per-lane offset and a per-access SGPR offset.  Column-strided accesses then need
+// one VGPR per lane instead of one 64-bit address per element, and accesses
+// outside the range read 0 / are dropped.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 bld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, int voff, int soff, float2 a) {
+  const u32x2 v = {__float_as_uint(a.x), __float_as_uint(a.y)};
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -172,21 +204,35 @@ template <int R, int SIGN> __device__ __forceinline__ void dft(float2* a) {
 // butterfly b in [0, N/R) reads x[b + q*N/R], multiplies input q by
 // W^(q*k) with k = b mod P and W = exp(SIGN 2 pi i / (P R)), runs DFT_R and
 // writes y[(b - k)*R + k + r*P].  Lane t runs butterflies b = t + T*u.
+//
+// Twiddles: the plan's table holds, for every stage after the first, P blocks
+// of R entries, block k = exp(-2 pi i q k / (P R)) for q < R (entry 0 is 1), so a
+// butterfly reads its R-1 factors contiguously (16-byte loads).
+//
+// LDS: with PP (ping-pong) two buffers are used alternately, so one barrier per
+// exchange suffices (the next exchange writes the other buffer, and the one after
+// that is ordered by the intervening barrier); without PP one buffer and two
+// barriers per exchange (half the LDS, so more workgroups per CU).  Addresses are
+// a per-lane base plus compile-time offsets, which the ds_read/ds_write immediate
+// offsets absorb.
 
-template <int N, int T, int SIGN, class Lay, int P, int... RS> struct Stages;
+template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, int... RS> struct Stages;
 
-template <int N, int T, int SIGN, class Lay, int P> struct Stages<N, T, SIGN, Lay, P> {
-  static __device__ __forceinline__ void run(float2*, float2*, const Lay&, int, const float2*) {}
+template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING> struct Stages<N, T, SIGN, Lay, PP, P, PING> {
+  static __device__ __forceinline__ void run(float2*, float2*, float2*, const Lay&, int, const float2*) {}
 };
 
-template <int N, int T, int SIGN, class Lay, int P, int R, int... RS> struct Stages<N, T, SIGN, Lay, P, R, RS...> {
+template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, int R, int... RS>
+struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
   static constexpr int E = N / T;
   static constexpr int U = E / R;
   static_assert(E % R == 0, "radix must divide elements per lane");
+  static_assert(T % 16 == 0, "lane count must be a multiple of 16");
   static constexpr bool LAST = sizeof...(RS) == 0;
 
-  static __device__ __forceinline__ void run(float2* x, float2* lds, const Lay& lay, int t, const float2* tw) {
-    constexpr int STRIDE = N / (P * R);  // twiddle table stride
+  static __device__ __forceinline__ void run(float2* x, float2* lds0, float2* lds1, const Lay& lay, int t,
+                                             const float2* __restrict__ tw) {
+    float2* buf = PING ? lds1 : lds0;
     float2 out[E];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -196,11 +242,24 @@ template <int N, int T, int SIGN, class Lay, int P, int R, int... RS> struct Sta
 #pragma unroll
       for (int q = 0; q < R; ++q) a[q] = x[u + q * U];
       if constexpr (P > 1) {
+        const float2* tk = tw + k * R;
+        float2 w[R];
+        if constexpr (R % 2 == 0) {
+          const float4* t4 = reinterpret_cast<const float4*>(__builtin_assume_aligned(tk, 16));
+#pragma unroll
+          for (int q = 0; q < R / 2; ++q) {
+            const float4 v = t4[q];
+            w[2 * q] = make_float2(v.x, v.y);
+            w[2 * q + 1] = make_float2(v.z, v.w);
+          }
+        } else {
+#pragma unroll
+          for (int q = 1; q < R; ++q) w[q] = tk[q];
+        }
 #pragma unroll
         for (int q = 1; q < R; ++q) {
-          float2 w = tw[q * k * STRIDE];
-          if (SIGN > 0) w.y = -w.y;
-          a[q] = cmul(a[q], w);
+          if (SIGN > 0) w[q].y = -w[q].y;
+          a[q] = cmul(a[q], w[q]);
         }
       }
       dft<R, SIGN>(a);
@@ -209,8 +268,14 @@ template <int N, int T, int SIGN, class Lay, int P, int R, int... RS> struct Sta
         for (int r = 0; r < R; ++r) out[u + r * U] = a[r];
       } else {
         const int j = (b - k) * R + k;
+        if constexpr (Lay::template linear_write<P, R>()) {
+          float2* wp = buf + lay(j);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[lay(j + r * P)] = a[r];
+          for (int r = 0; r < R; ++r) wp[Lay::off(r * P)] = a[r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) buf[lay(j + r * P)] = a[r];
+        }
       }
     }
     if constexpr (LAST) {
@@ -218,31 +283,58 @@ template <int N, int T, int SIGN, class Lay, int P, int R, int... RS> struct Sta
       for (int v = 0; v < E; ++v) x[v] = out[v];
     } else {
       __syncthreads();
+      const float2* rp = buf + lay(t);
 #pragma unroll
-      for (int v = 0; v < E; ++v) x[v] = lds[lay(t + T * v)];
-      __syncthreads();
-      Stages<N, T, SIGN, Lay, P * R, RS...>::run(x, lds, lay, t, tw);
+      for (int v = 0; v < E; ++v) x[v] = rp[Lay::off(T * v)];
+      if constexpr (!PP) __syncthreads();
+      Stages<N, T, SIGN, Lay, PP, P * R, PP ? 1 - PING : 0, RS...>::run(x, lds0, lds1, lay, t,
+                                                                       tw + (P > 1 ? P * R : 0));
     }
   }
 };
 
 // Radix plans: lane count T and radix list per supported length.
+template <int N_, int T_, int... RS> struct PlanDef {
+  static constexpr int N = N_;
+  static constexpr int T = T_;
+  static constexpr int NRAD = sizeof...(RS);
+  static constexpr int EXCH = NRAD - 1;  // LDS exchanges per transform
+  template <int S, class L, bool PP, int START> using St = Stages<N, T, S, L, PP, 1, START, RS...>;
+  static constexpr int radix(int i) {
+    constexpr int r[] = {RS...};
+    return r[i];
+  }
+};
 template <int N> struct Plan;
-template <> struct Plan<64> { static constexpr int T = 16; template <int S, class L> using St = Stages<64, 16, S, L, 1, 4, 4, 4>; };
-template <> struct Plan<128> { static constexpr int T = 16; template <int S, class L> using St = Stages<128, 16, S, L, 1, 8, 4, 4>; };
-template <> struct Plan<256> { static constexpr int T = 16; template <int S, class L> using St = Stages<256, 16, S, L, 1, 16, 16>; };
-template <> struct Plan<512> { static constexpr int T = 32; template <int S, class L> using St = Stages<512, 32, S, L, 1, 16, 16, 2>; };
-template <> struct Plan<768> { static constexpr int T = 32; template <int S, class L> using St = Stages<768, 32, S, L, 1, 8, 3, 8, 4>; };
-template <> struct Plan<1024> { static constexpr int T = 64; template <int S, class L> using St = Stages<1024, 64, S, L, 1, 16, 16, 4>; };
-template <> struct Plan<1536> { static constexpr int T = 64; template <int S, class L> using St = Stages<1536, 64, S, L, 1, 8, 3, 8, 8>; };
-template <> struct Plan<2048> { static constexpr int T = 128; template <int S, class L> using St = Stages<2048, 128, S, L, 1, 16, 16, 8>; };
+template <> struct Plan<64> : PlanDef<64, 16, 4, 4, 4> {};
+template <> struct Plan<128> : PlanDef<128, 16, 8, 4, 4> {};
+template <> struct Plan<256> : PlanDef<256, 16, 16, 16> {};
+template <> struct Plan<512> : PlanDef<512, 32, 16, 16, 2> {};
+template <> struct Plan<768> : PlanDef<768, 32, 8, 3, 8, 4> {};
+template <> struct Plan<1024> : PlanDef<1024, 64, 16, 16, 4> {};
+template <> struct Plan<1536> : PlanDef<1536, 64, 8, 3, 8, 8> {};
+template <> struct Plan<2048> : PlanDef<2048, 128, 16, 16, 8> {};
+
+// Number of twiddle entries of a plan's table.
+template <int N> __host__ __device__ constexpr int twiddle_len() {
+  int p = 1, len = 0;
+  for (int i = 0; i < Plan<N>::NRAD; ++i) {
+    if (p > 1) len += p * Plan<N>::radix(i);
+    p *= Plan<N>::radix(i);
+  }
+  return len;
+}
 
 // x holds elements t + T*v of one transform; after the call it holds the
-// corresponding outputs.
-template <int N, int SIGN, class Lay>
-__device__ __forceinline__ void block_fft(float2* x, float2* lds, const Lay& lay, int t, const float2* tw) {
-  Plan<N>::template St<SIGN, Lay>::run(x, lds, lay, t, tw);
+// corresponding outputs.  START selects the first LDS buffer: a transform that
+// follows another must start on the buffer the previous one did not use last.
+template <int N, int SIGN, bool PP, int START = 0, class Lay>
+__device__ __forceinline__ void block_fft(float2* x, float2* lds0, float2* lds1, const Lay& lay, int t,
+                                          const float2* tw) {
+  Plan<N>::template St<SIGN, Lay, PP, START>::run(x, lds0, lds1, lay, t, tw);
 }
+// START for a transform that follows one started on buffer `prev`.
+template <int N, bool PP> constexpr int next_start(int prev) { return PP ? (prev + Plan<N>::EXCH) % 2 : 0; }
 
 // LDS layouts.  Row transforms: one padding slot per 16 elements so that the
 // first stage's stride-R writes spread over the banks.  Column transforms:
@@ -251,10 +343,15 @@ __device__ __forceinline__ void block_fft(float2* x, float2* lds, const Lay& lay
 struct RowLay {
   int base;
   __device__ __forceinline__ int operator()(int e) const { return base + e + (e >> 4); }
+  // slot(e + k) - slot(e) for k a multiple of 16
+  static __device__ __forceinline__ constexpr int off(int k) { return k + (k >> 4); }
+  template <int P, int R> static constexpr bool linear_write() { return P % 16 == 0 || (P == 1 && R % 16 == 0); }
 };
 template <int C> struct ColLay {
   int c;
   __device__ __forceinline__ int operator()(int e) const { return e * C + c; }
+  static __device__ __forceinline__ constexpr int off(int k) { return k * C; }
+  template <int P, int R> static constexpr bool linear_write() { return true; }
 };
 template <int N> __host__ __device__ constexpr int row_lds_len() { return N + N / 16; }
 

@@ -12,20 +12,66 @@
 #include "fft_block.h"
 #include "kernels.h"
 #include "launch.h"
+#include "unit_round.h"
+
+#include <cmath>
+
+// Tuning knobs (compile-time).
+#ifndef TOTEM_ROWS_MINWAVES
+#define TOTEM_ROWS_MINWAVES 1  // minimum waves per SIMD the rows pass is compiled for
+#endif
+#ifndef TOTEM_COLS_MINWAVES
+#define TOTEM_COLS_MINWAVES 1
+#endif
+#ifndef TOTEM_COLS_C1024
+#define TOTEM_COLS_C1024 4     // columns per workgroup of the 1024-point columns pass
+#endif
+#ifndef TOTEM_ROWS_PP
+#define TOTEM_ROWS_PP 0        // rows pass: ping-pong LDS (1) or one buffer (0)
+#endif
+#ifndef TOTEM_COLS_PP
+#define TOTEM_COLS_PP 0
+#endif
+#ifndef TOTEM_XCD_REMAP
+#define TOTEM_XCD_REMAP 1      // XCD-aware workgroup -> (probe, row/column group) mapping
+#endif
 
 namespace totem {
 
 // ------------------------------------------------------------------------- helpers
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// With the remap, XCD x works through the groups [x*G/8, (x+1)*G/8) of every probe
+// in order: neighbouring column groups (which share 128-byte lines) run back to back
+// on one L2, and each L2 only ever sees 1/8 of the propagator / a band of the
+// transmission slice.  Any G divisible by 8 gives a bijection; otherwise the plain
+// probe-major order is used.
+__device__ __forceinline__ void xcd_map(int bid, int G, int& item, int& g) {
+#if TOTEM_XCD_REMAP
+  if ((G & 7) == 0) {
+    const int xcd = bid & 7, k = bid >> 3, gp = G >> 3;
+    item = k / gp;
+    g = xcd * gp + (k - item * gp);
+    return;
+  }
+#endif
+  item = bid / G;
+  g = bid - item * G;
+}
 
 template <int N> struct RowCfg {
   static constexpr int T = Plan<N>::T;
   static constexpr int RW = (256 / T) > 0 ? 256 / T : 1;  // rows per workgroup
   static constexpr int THREADS = RW * T;
   static constexpr int LL = row_lds_len<N>();
+  static constexpr bool PP = TOTEM_ROWS_PP != 0;
+  static constexpr int NBUF = PP ? 2 : 1;
 };
 template <int N> struct ColCfg {
+  static constexpr bool PP = TOTEM_COLS_PP != 0;
+  static constexpr int NBUF = PP ? 2 : 1;
   static constexpr int T = Plan<N>::T;
-  static constexpr int C = (N <= 512) ? 16 : (N >= 1536 ? 4 : 8);  // columns per workgroup
+  static constexpr int C = (N <= 512) ? 16 : (N >= 1536 ? 4 : (N == 1024 ? TOTEM_COLS_C1024 : 8));  // columns per workgroup
   static constexpr int THREADS = T * C;
 };
 
@@ -38,14 +84,15 @@ __device__ __forceinline__ double wave_sum(double v) {
 // ------------------------------------------------------------------------ pass R
 
 template <int N>
-__global__ void __launch_bounds__(RowCfg<N>::THREADS)
+__global__ void __launch_bounds__(RowCfg<N>::THREADS, TOTEM_ROWS_MINWAVES)
 k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, float2* __restrict__ psi,
             int step, const float2* __restrict__ tw) {
   using Cfg = RowCfg<N>;
   constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, G = N / RW;
-  __shared__ float2 lds[RW * Cfg::LL];
-  const int item = blockIdx.x / G;
-  const int rg = blockIdx.x - item * G;
+  __shared__ float2 lds[Cfg::NBUF * RW * Cfg::LL];
+  float2* const lds1 = lds + (Cfg::NBUF - 1) * RW * Cfg::LL;
+  int item, rg;
+  xcd_map(blockIdx.x, G, item, rg);
   const ItemDev it = items[item];
   const JobDev& jb = jobs[it.job];
   const int slice = step - it.start_step;
@@ -55,15 +102,15 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   const int row = rg * RW + lr;
   const RowLay lay{lr * Cfg::LL};
   float2 x[E];
-  float2* w = psi + (size_t)it.slot * N * N + (size_t)row * N;
+  float2* w = psi + (size_t)it.slot * N * N + (size_t)row * N + t;
   if (slice == 0) {
-    const float2* p = it.probe + (size_t)row * N;
+    const float2* p = it.probe + (size_t)row * N + t;
 #pragma unroll
-    for (int v = 0; v < E; ++v) x[v] = p[t + T * v];
+    for (int v = 0; v < E; ++v) x[v] = gld(p + T * v);
   } else {
 #pragma unroll
-    for (int v = 0; v < E; ++v) x[v] = w[t + T * v];
-    block_fft<N, +1>(x, lds, lay, t, tw);
+    for (int v = 0; v < E; ++v) x[v] = gld(w + T * v);
+    block_fft<N, +1, Cfg::PP, 0>(x, lds, lds1, lay, t, tw);
     const float s = 1.0f / (float)(N * N);
 #pragma unroll
     for (int v = 0; v < E; ++v) x[v] = cscale(x[v], s);
@@ -76,11 +123,11 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   for (int v = 0; v < E; ++v) {
     int pc = it.c0 + t + T * v;
     if (pc >= np) pc -= np;
-    x[v] = cmul(x[v], trow[pc]);
+    x[v] = cmul(x[v], gld(trow + pc));
   }
-  block_fft<N, -1>(x, lds, lay, t, tw);
+  block_fft<N, -1, Cfg::PP, next_start<N, Cfg::PP>(0)>(x, lds, lds1, lay, t, tw);
 #pragma unroll
-  for (int v = 0; v < E; ++v) w[t + T * v] = x[v];
+  for (int v = 0; v < E; ++v) gst(w + T * v, x[v]);
 }
 
 // ------------------------------------------------------------------------ pass C
@@ -101,7 +148,7 @@ __device__ __forceinline__ void detector_partial(const float2* x, const double* 
     const double* m = masks + (size_t)mrow * N + mcol;
 #pragma unroll
     for (int a = 0; a < kMaxAper; ++a)
-      if (a < A) acc[a] += inten * m[(size_t)a * N * N];
+      if (a < A) acc[a] += inten * gldd(m + (size_t)a * N * N);
   }
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   constexpr int NW = (T * C + 63) / 64;
@@ -121,17 +168,19 @@ __device__ __forceinline__ void detector_partial(const float2* x, const double* 
   __syncthreads();
 }
 
-template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+// DET = false: no item of the launch is at a detector slice (the lean variant).
+template <int N, bool DET>
+__global__ void __launch_bounds__(ColCfg<N>::THREADS, TOTEM_COLS_MINWAVES)
 k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, float2* __restrict__ psi,
             int step, const float2* __restrict__ tw, double* __restrict__ partial, int lmax1, int amax) {
   using Cfg = ColCfg<N>;
   constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C;
   constexpr int NW = (Cfg::THREADS + 63) / 64;
-  __shared__ float2 lds[N * C];
-  __shared__ double red[NW * kMaxAper];
-  const int item = blockIdx.x / G;
-  const int cg = blockIdx.x - item * G;
+  __shared__ float2 lds[Cfg::NBUF * N * C];
+  __shared__ double red[DET ? NW * kMaxAper : 1];
+  float2* const lds1 = lds + (Cfg::NBUF - 1) * N * C;
+  int item, cg;
+  xcd_map(blockIdx.x, G, item, cg);
   const ItemDev it = items[item];
   const JobDev& jb = jobs[it.job];
   const int slice = step - it.start_step;
@@ -141,29 +190,34 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   const int col = cg * C + c;
   const ColLay<C> lay{c};
   float2 x[E];
-  float2* w = psi + (size_t)it.slot * N * N + col;
+  constexpr unsigned kBytes = (unsigned)N * N * sizeof(float2);
+  const auto wr = buf_rsrc(psi + (size_t)it.slot * N * N, kBytes);
+  const int voff = (t * N + col) * (int)sizeof(float2);
+  constexpr int kStep = T * N * (int)sizeof(float2);
 #pragma unroll
-  for (int v = 0; v < E; ++v) x[v] = w[(size_t)(t + T * v) * N];
-  block_fft<N, -1>(x, lds, lay, t, tw);
-  const float2* hp = jb.prop + col;
+  for (int v = 0; v < E; ++v) x[v] = bld(wr, voff, v * kStep);
+  block_fft<N, -1, Cfg::PP, 0>(x, lds, lds1, lay, t, tw);
+  const auto hr = buf_rsrc(jb.prop, kBytes);
 #pragma unroll
-  for (int v = 0; v < E; ++v) x[v] = cmul(x[v], hp[(size_t)(t + T * v) * N]);
-  const int mid = jb.mid_of_slice ? jb.mid_of_slice[slice] : -1;
-  double* pbase = partial + (size_t)it.slot * lmax1 * amax * G;
-  if (mid >= 0) {
-    for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
-      detector_partial<N, E, T, C>(x, jb.aper2 + (size_t)a0 * N * N, false, min(kMaxAper, jb.A - a0), t, col, red,
-                                   pbase + ((size_t)mid * amax + a0) * G, G, cg);
+  for (int v = 0; v < E; ++v) x[v] = cmul(x[v], bld(hr, voff, v * kStep));
+  if constexpr (DET) {
+    const int mid = jb.mid_of_slice ? jb.mid_of_slice[slice] : -1;
+    double* pbase = partial + (size_t)it.slot * lmax1 * amax * G;
+    if (mid >= 0) {
+      for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
+        detector_partial<N, E, T, C>(x, jb.aper2 + (size_t)a0 * N * N, false, min(kMaxAper, jb.A - a0), t, col,
+                                     red, pbase + ((size_t)mid * amax + a0) * G, G, cg);
+    }
+    if (slice == jb.S - 1) {
+      for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
+        detector_partial<N, E, T, C>(x, jb.aper + (size_t)a0 * N * N, true, min(kMaxAper, jb.A - a0), t, col, red,
+                                     pbase + ((size_t)jb.L * amax + a0) * G, G, cg);
+      return;
+    }
   }
-  if (slice == jb.S - 1) {
-    for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
-      detector_partial<N, E, T, C>(x, jb.aper + (size_t)a0 * N * N, true, min(kMaxAper, jb.A - a0), t, col, red,
-                                   pbase + ((size_t)jb.L * amax + a0) * G, G, cg);
-    return;
-  }
-  block_fft<N, +1>(x, lds, lay, t, tw);
+  block_fft<N, +1, Cfg::PP, next_start<N, Cfg::PP>(0)>(x, lds, lds1, lay, t, tw);
 #pragma unroll
-  for (int v = 0; v < E; ++v) w[(size_t)(t + T * v) * N] = x[v];
+  for (int v = 0; v < E; ++v) bst(wr, voff, v * kStep, x[v]);
 }
 
 // Fixed-order Kahan sum of the detector partials of the listed items.
@@ -207,7 +261,7 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS) k_fft_rows(float2* __restr
   const bool ok = row < rows;
 #pragma unroll
   for (int v = 0; v < E; ++v) x[v] = ok ? w[t + T * v] : make_float2(0.f, 0.f);
-  block_fft<N, SIGN>(x, lds, RowLay{lr * Cfg::LL}, t, tw);
+  block_fft<N, SIGN, false, 0>(x, lds, lds, RowLay{lr * Cfg::LL}, t, tw);
   if (ok) {
 #pragma unroll
     for (int v = 0; v < E; ++v) w[t + T * v] = x[v];
@@ -232,7 +286,7 @@ k_fft_cols(const float2* __restrict__ src, float2* __restrict__ dst, int epi, fl
   float2 x[E];
 #pragma unroll
   for (int v = 0; v < E; ++v) x[v] = s[(size_t)(t + T * v) * N];
-  block_fft<N, SIGN>(x, lds, ColLay<C>{c}, t, tw);
+  block_fft<N, SIGN, false, 0>(x, lds, lds, ColLay<C>{c}, t, tw);
 #pragma unroll
   for (int v = 0; v < E; ++v) {
     float2 y = x[v];
@@ -370,11 +424,15 @@ template <int N> void launch_pass_rows(hipStream_t st, const ItemDev* items, int
   hipLaunchKernelGGL(k_pass_rows<N>, grid, dim3(Cfg::THREADS), 0, st, items, jobs, psi, step, tw_table(N));
 }
 template <int N> void launch_pass_cols(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
-                                       float2* psi, int step, double* partial, int lmax1, int amax) {
+                                       float2* psi, int step, double* partial, int lmax1, int amax, bool det) {
   using Cfg = ColCfg<N>;
   const dim3 grid((unsigned)(n_items * (N / Cfg::C)));
-  hipLaunchKernelGGL(k_pass_cols<N>, grid, dim3(Cfg::THREADS), 0, st, items, jobs, psi, step, tw_table(N),
-                     partial, lmax1, amax);
+  if (det)
+    hipLaunchKernelGGL((k_pass_cols<N, true>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, psi, step, tw_table(N),
+                       partial, lmax1, amax);
+  else
+    hipLaunchKernelGGL((k_pass_cols<N, false>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, psi, step,
+                       tw_table(N), partial, lmax1, amax);
 }
 template <int N> void launch_fft2d(hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi,
                                    float sigma) {
@@ -391,7 +449,29 @@ template <int N> void launch_fft2d(hipStream_t st, float2* data, float2* dst, in
 }
 template <int N> int col_groups() { return N / ColCfg<N>::C; }
 
+template <int N> void twiddle_host_n(std::vector<float2>& out) {
+  out.clear();
+  int p = 1;
+  for (int i = 0; i < Plan<N>::NRAD; ++i) {
+    const int r = Plan<N>::radix(i);
+    if (p > 1)
+      for (int k = 0; k < p; ++k)
+        for (int q = 0; q < r; ++q) {
+          const double a = -2.0 * M_PI * (double)q * (double)k / (double)(p * r);
+          out.push_back(unit_float2(std::cos(a), std::sin(a)));
+        }
+    p *= r;
+  }
+}
+
 #define TOTEM_FOR_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(1536) X(2048)
+
+void twiddle_host(int n, std::vector<float2>& out) {
+  out.clear();
+#define X(S) if (n == S) { twiddle_host_n<S>(out); return; }
+  TOTEM_FOR_SIZES(X)
+#undef X
+}
 
 bool size_supported(int n) {
 #define X(S) if (n == S) return true;
@@ -414,8 +494,8 @@ void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const J
 }
 
 void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step,
-               double* partial, int lmax1, int amax) {
-#define X(S) if (n == S) { launch_pass_cols<S>(st, items, n_items, jobs, psi, step, partial, lmax1, amax); return; }
+               double* partial, int lmax1, int amax, bool det) {
+#define X(S) if (n == S) { launch_pass_cols<S>(st, items, n_items, jobs, psi, step, partial, lmax1, amax, det); return; }
   TOTEM_FOR_SIZES(X)
 #undef X
 }

@@ -3,18 +3,25 @@
 #include <hip/hip_runtime.h>
 #include "kernels.h"
 
+#include <vector>
+
 namespace totem {
 
-// FP32 twiddle table exp(-2 pi i m / n), m < n, generated in FP64; device
-// pointer for the current device, created on first use and kept for the process.
+// Device pointer (current device) of the n-point plan's twiddle table, created on
+// first use from twiddle_host() and kept for the process.
 const float2* tw_table(int n);
+// Host copy of the n-point plan's stage twiddle tables (see fft_block.h), FP64
+// values rounded to unit-modulus FP32 pairs.
+void twiddle_host(int n, std::vector<float2>& out);
 
 bool size_supported(int n);   // FFT lengths with a compiled radix plan
 int detector_groups(int n);   // column groups per probe in k_pass_cols
 
 void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step);
+// det: some item of the launch may be at a detector slice (mid-slice or exit face);
+// false selects the variant without detector code.
 void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step,
-               double* partial, int lmax1, int amax);
+               double* partial, int lmax1, int amax, bool det);
 void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs,
                 const double* partial, int groups, int lmax1, int amax);
 // In-place batched 2-D FFT of `batch` n x n complex64 matrices; the column pass

@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
-CORE = os.path.join(LIB_DIR, "libtotem.so")
+CORE = os.environ.get("TOTEM_LIB") or os.path.join(LIB_DIR, "libtotem.so")
 VARIANT_LIBS = {
     "waveCudalobato": "libwaveCudalobato.so",
     "waveCudaPengAndCorr": "libwaveCudaPengAndCorr.so",
