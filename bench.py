@@ -65,6 +65,33 @@ def cpu_baseline(t_dev, inp, positions, cores):
     return len(positions) * cfg.n_slices / wall, wall, np.asarray(res)
 
 
+def run_serving(n_requests, rate, seed, arena_gib, budget_gib):
+    """Poisson arrivals of mixed-size STEM requests (grids 256^2 50%, 512^2 30%, 1024^2 15%,
+    2048^2 5%; 10-120 slices; scans 8x8-64x64; Lobato or Peng; SrTiO3 or amorphous C) through the
+    served engine.  Returns the latency percentiles and the aggregate probe.slice/s."""
+    from totem_amd import engine, serve, stem
+    rng = np.random.default_rng(seed)
+    reqs, t, work = [], 0.0, 0
+    mix = {}
+    for k in range(n_requests):
+        cfg, kind = serve.mixed_request(rng)
+        t += rng.exponential(1.0 / rate)
+        reqs.append(serve.Request(rid=k, arrival=t, inp=stem.prepare(cfg), kind=kind))
+        work += cfg.depth * cfg.n_slices
+        mix["%d" % cfg.n] = mix.get("%d" % cfg.n, 0) + 1
+    srv = engine.Server(int(arena_gib * 2**30), 8, 16)
+    try:
+        wall, peak = serve.serve(reqs, srv, hbm_budget=int(budget_gib * 2**30))
+    finally:
+        srv.close()
+    lat = np.array([r.latency for r in reqs])
+    return {"requests": n_requests, "arrival_rate_per_s": rate, "grids": mix,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "max_latency_s": float(lat.max()), "wall_s": wall, "probe_slices": int(work),
+            "throughput_probe_slices_per_s": work / wall, "peak_resident_GiB": peak / 2**30,
+            "arena_GiB": arena_gib, "hbm_budget_GiB": budget_gib, "seed": seed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +103,11 @@ def main():
     ap.add_argument("--cpu-probes", type=int, default=16)
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests of the mixed serving load (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-traffic kernel for "
+                         "scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (multi-GPU sharding is design-only); "
@@ -117,6 +149,13 @@ def main():
     ws_bytes = engine.stem_workspace_bytes(job, batch)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
+    if args.calibrate_copy_mib > 0:
+        src = torch.ones(args.calibrate_copy_mib << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def build(r):
         corr = cfg.paraflag == "n"
@@ -158,18 +197,20 @@ def main():
 
     # plain form: same kernels, one probe per launch sequence, no graph
     pp = min(args.plain_probes, D)
-    det_p = torch.zeros((1, A, M * pp), dtype=torch.float64, device=dev)
-    job_p = engine.stem_job(N, NP, S, t.data_ptr(), probes.data_ptr(), M, prop.data_ptr(), aper.data_ptr(), A, pp,
-                            cfg.width_red, cfg.step, cfg.begin_row - 1, cfg.begin_col - 1, det_p.data_ptr(),
-                            aper2_ptr=aper2.data_ptr(), wrap=True)
-    engine.stem_run(job_p, 1, workspace_ptr=ws.data_ptr(), workspace_bytes=ws_bytes, stream=stream)
-    torch.cuda.synchronize()
-    tp = time.perf_counter()
-    engine.stem_run(job_p, 1, workspace_ptr=ws.data_ptr(), workspace_bytes=ws_bytes, stream=stream)
-    torch.cuda.synchronize()
-    plain_wall = time.perf_counter() - tp
-    plain_value = M * pp * S / plain_wall
-    plain_match = bool(np.array_equal(det_p.cpu().numpy()[0], det_ref[:, :M * pp]))
+    plain_value = plain_match = None
+    if pp > 0:
+        det_p = torch.zeros((1, A, M * pp), dtype=torch.float64, device=dev)
+        job_p = engine.stem_job(N, NP, S, t.data_ptr(), probes.data_ptr(), M, prop.data_ptr(), aper.data_ptr(), A,
+                                pp, cfg.width_red, cfg.step, cfg.begin_row - 1, cfg.begin_col - 1, det_p.data_ptr(),
+                                aper2_ptr=aper2.data_ptr(), wrap=True)
+        engine.stem_run(job_p, 1, workspace_ptr=ws.data_ptr(), workspace_bytes=ws_bytes, stream=stream)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        engine.stem_run(job_p, 1, workspace_ptr=ws.data_ptr(), workspace_bytes=ws_bytes, stream=stream)
+        torch.cuda.synchronize()
+        plain_wall = time.perf_counter() - tp
+        plain_value = M * pp * S / plain_wall
+        plain_match = bool(np.array_equal(det_p.cpu().numpy()[0], det_ref[:, :M * pp]))
 
     # roofline of the dominant kernel (HIP events on the launch stream)
     n2 = float(N) * N
@@ -203,7 +244,8 @@ def main():
         "data": "synthetic (seeded specimen and phonon snapshots)",
         "config": {"workload": name, "grid": N, "slices": S, "probes_per_step": M * D,
                    "phonon_configs_timed": args.steps, "paraflag": cfg.paraflag, "graph": bool(args.graph)},
-        "plain": {"value": plain_value, "probes": pp, "speedup_batched_vs_plain": value / plain_value,
+        "plain": {"value": plain_value, "probes": pp,
+                  "speedup_batched_vs_plain": value / plain_value if plain_value else None,
                   "bit_identical_to_batched": plain_match},
         "potential_build_s_per_config": build_s,
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -222,6 +264,10 @@ def main():
                                          "oracle (propagation only, transmission taken from the GPU build), "
                                          "%.1f s wall" % (len(positions), S, n_cfg - 1, cwall)}
         out["oracle_max_rel_err"] = rel
+    if args.serve_requests > 0:
+        del ws, t, probes, det
+        torch.cuda.empty_cache()
+        out["serving"] = run_serving(args.serve_requests, args.serve_rate, seed=1005, arena_gib=64, budget_gib=160)
     print(json.dumps(out))
 
 

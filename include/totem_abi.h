@@ -130,6 +130,10 @@ typedef struct totem_stem_job {
   int wrap;              /* 1: periodic window (mod np); 0: window must lie inside the potential */
   double *det;           /* device [(L+1)][A][M*D]: mid layers 0..L-1, exit face at L;
                             column m*D + d (no focal-spread weighting applied) */
+  void *exit_wave;       /* device complex64 [M*D][n][n] or NULL: the exit-face wave in reciprocal
+                            space, Psi(k) = FFT2(psi) (zero frequency at [0][0], unnormalised),
+                            i.e. what the reference's final forward FFT produces before its
+                            quadrant swap (cuda_source_lobato.cu:1405-1415) */
 } totem_stem_job;
 
 #define TOTEM_RUN_GRAPH 1     /* capture the slice loop of a batch in a hipGraph and replay it */
@@ -146,6 +150,28 @@ int totem_stem_run(const totem_stem_job *job, int batch, int flags, void *worksp
 /* Accumulated durations (ms) and launch counts of the rows and columns passes
  * over TOTEM_RUN_TIMING runs on the calling thread; reset != 0 clears them after reading. */
 void totem_kernel_times(int reset, double *rows_ms, long *rows_n, double *cols_ms, long *cols_n);
+
+/* ------------------------------------------------------------- served engine
+ * Continuous batching of many device STEM jobs (no counterpart in the reference,
+ * which runs one myProjectfun call at a time): jobs of any supported grid side are
+ * submitted at any time, every step advances each in-flight probe by one slice,
+ * probes of all jobs with the same grid side share launches, and a job's probes
+ * are admitted into wave slots (first come, first served) as slots free up.  A
+ * job's arrays must stay valid, and be ready on the device, from submit until
+ * poll reports it; its det (and exit_wave) are then final.  Results equal those
+ * of the same job run alone.  One host thread per server. */
+typedef struct totem_server totem_server;
+/* arena_bytes of HBM for in-flight waves (a page holds one 2048^2 slot or several
+ * smaller ones); jobs may use up to max_aper detectors and max_mid mid slices. */
+int totem_server_create(size_t arena_bytes, int max_aper, int max_mid, totem_server **out);
+int totem_server_submit(totem_server *srv, const totem_stem_job *job, long long *id);
+/* Admits waiting probes and advances every in-flight probe by one slice
+ * (asynchronous); in_flight receives the probes in flight after admission. */
+int totem_server_step(totem_server *srv, int *in_flight);
+/* Ids of up to cap finished jobs (each reported once). */
+int totem_server_poll(totem_server *srv, long long *ids, int cap, int *n_done);
+int totem_server_sync(totem_server *srv);
+int totem_server_destroy(totem_server *srv);
 
 /* Batched in-place 2-D FFT of `batch` n x n complex64 matrices on the device
  * (sign -1 forward, +1 inverse, unnormalised).  Test/diagnostic entry point. */

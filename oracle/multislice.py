@@ -23,8 +23,9 @@ specimen; focal-series linearity; scan/translation equivalence.
 import numpy as np
 
 
-def stem_position(t, probe, gfsf, psf_fft, aper_shifted, r0, c0, midslice=(), periodic=False):
-    """One scan position, all focal members: (final [A], mid [L, A]) in FP64.
+def stem_position(t, probe, gfsf, psf_fft, aper_shifted, r0, c0, midslice=(), periodic=False, waves=False):
+    """One scan position, all focal members: (final [A], mid [L, A]) in FP64, and with
+    ``waves`` also the exit waves fft2(psi) [M, n, n] the final detector integrates (``:74``).
 
     ``periodic`` takes the window modulo the potential side (the periodic extension
     of the reciprocal-space potential); the reference crops without wrapping."""
@@ -41,6 +42,7 @@ def stem_position(t, probe, gfsf, psf_fft, aper_shifted, r0, c0, midslice=(), pe
     mids = list(midslice)
     res = np.zeros(A)
     mid = np.zeros((len(mids), A))
+    exits = []
     for nn in range(len(gfsf)):
         wave = probe[nn].astype(np.complex128)
         for k in range(1, S + 1):
@@ -53,8 +55,11 @@ def stem_position(t, probe, gfsf, psf_fft, aper_shifted, r0, c0, midslice=(), pe
                     mid[oo, kk] += gfsf[nn] * np.sum(np.abs(wave * aper_shifted[kk]) ** 2)
             wave = np.fft.ifft2(wave)
         wave = np.fft.fft2(wave)
+        exits.append(wave)
         for kk in range(A):
             res[kk] += gfsf[nn] * np.sum(np.abs(wave * aper_shifted[kk]) ** 2)
+    if waves:
+        return res, mid, np.asarray(exits)
     return res, mid
 
 

@@ -1,0 +1,95 @@
+"""Writes the golden fixtures of the STEM path: ``python tests/golden/make_golden.py``.
+
+Inputs are the reference's own specimen file ``allresul.mat`` (a 791-atom SrTiO3
+model: ``x_new`` = [Z x y z valence ionicity B occ], ``slicethick`` = 41 slice
+faces 0.823175 A apart, ``eachthick``), read as data with ``scipy.io.loadmat``
+from the reference checkout and stored here as ``allresul_specimen.npz``.  The
+expected outputs are those of the FP64 oracle (``oracle/``) on the cases of
+``CASES`` (Lobato at 200 kV; Peng + correction at 300 kV with a 3-member focal
+series), both on a 256^2 potential over 36 A with a 128^2 probe window:
+
+* potential slices: a 64 x 64 crop (rows/columns 96..159) of V at slices 0, 20
+  and 40 (complex64) plus, for every slice, sum V and sum |V|^2 (FP64);
+* the mex outputs of a 4 x 4 scan: ``myresul`` [D, A] and ``mid_ceng_mat`` [D*A, L].
+
+Only this script reads the reference checkout; the tests read the ``.npz`` files.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference/allresul.mat"
+
+CROP = (slice(96, 160), slice(96, 160))
+V_SLICES = (0, 20, 40)
+CASES = {
+    "lobato": dict(paraflag="l", kv=200.0, alpha_mrad=20.0, detectors_mrad=((30.0, 59.0), (0.0, 10.0)),
+                   mid_slices=(10, 25), focal_members=1, focal_rms_nm=0.0, aberrations=None),
+    "pengcorr": dict(paraflag="n", kv=300.0, alpha_mrad=20.0, detectors_mrad=((25.0, 46.0), (0.0, 10.0)),
+                     mid_slices=(20,), focal_members=3, focal_rms_nm=3.0, aberrations=(-5.0, 1.0)),
+}
+
+
+def config(spec, name):
+    """StemConfig of golden case ``name`` on the stored specimen (no reference access)."""
+    sys.path.insert(0, ROOT)
+    from totem_amd import optics, stem
+    c = dict(CASES[name])
+    ab = c.pop("aberrations")
+    ab = optics.Aberrations(C1=ab[0], Cs_um=ab[1]) if ab else None
+    dz = float(spec["eachthick"])
+    faces = np.asarray(spec["slicethick"], dtype=np.float64)
+    cfg = stem.StemConfig(atoms=np.asarray(spec["atoms"], dtype=np.float64), np_side=256, dx=36.0 / 256, n=128,
+                          slice_thickness=dz, n_slices=len(faces), slice_origin=float(faces[0]), aberrations=ab,
+                          width_red=4, hight_red=4, step=6, begin_row=40, begin_col=44, **c)
+    assert np.allclose(stem.slice_boundaries(cfg), faces, rtol=0, atol=1e-9)
+    return cfg
+
+
+def oracle_outputs(cfg):
+    """(V [S, Np, Np] complex128, myresul [D, A], mid [D*A, L]) from the FP64 oracle."""
+    sys.path.insert(0, ROOT)
+    from oracle import multislice as oms
+    from oracle import potential as opot
+    from totem_amd import stem
+    inp = stem.prepare(cfg)
+    kw = {}
+    if cfg.paraflag == "n":
+        kw = dict(ele_n_corr=inp.ele_n_corr, corr_matrix=inp.corr_matrix, series_n_corr=inp.series_n_corr)
+    V, t = opot.potential_slices(cfg.np_side, inp.ele_n, None, inp.ele_n_i, None, inp.series_n, inp.series_n_i,
+                                 inp.s2, inp.gx, inp.gy, inp.sigma, inp.parameter, inp.aper_true, cfg.paraflag, **kw)
+    res, mid = oms.stem_cpu(t, inp.probe, inp.gfsf, inp.psf, cfg.begin_row, cfg.begin_col, cfg.width_red,
+                            cfg.hight_red, cfg.step, inp.aper, tuple(sorted(cfg.mid_slices)))
+    return V, res, mid
+
+
+def summarise(V, res, mid):
+    return {"v_crop": V[(slice(None),) + CROP][list(V_SLICES)].astype(np.complex64),
+            "v_sum": V.sum(axis=(1, 2)), "v_norm2": (np.abs(V) ** 2).sum(axis=(1, 2)),
+            "myresul": res, "mid": mid}
+
+
+def main():
+    import scipy.io as sio
+    d = sio.loadmat(REF)["allresul"][0, 0]
+    atoms = np.asarray(d["x_new"], dtype=np.float64)
+    faces = np.asarray(d["slicethick"], dtype=np.float64).ravel()
+    dz = float(np.asarray(d["eachthick"]).ravel()[0])
+    gap = np.min(np.abs(atoms[:, 3][:, None] - faces[None, :]))
+    assert gap > 1e-6, "an atom sits on a slice face"
+    spec = {"atoms": atoms, "slicethick": faces, "eachthick": np.float64(dz)}
+    np.savez_compressed(os.path.join(HERE, "allresul_specimen.npz"), **spec)
+    out = {}
+    for name in CASES:
+        V, res, mid = oracle_outputs(config(spec, name))
+        for k, v in summarise(V, res, mid).items():
+            out["%s_%s" % (name, k)] = v
+        print(name, "myresul", res.ravel()[:4], "mid", mid.shape, flush=True)
+    np.savez_compressed(os.path.join(HERE, "allresul_golden.npz"), **out)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,76 @@
+"""Golden fixtures on the reference's own specimen file (``allresul.mat``: 791-atom SrTiO3,
+41 slices of 0.823175 A), written by ``tests/golden/make_golden.py``.
+
+CPU: the oracle and the host set-up reproduce the stored FP64 outputs.
+GPU: the engine, through ``myProjectfun`` and ``totem_build_transmission``, meets the
+stated tolerances against them (potential <= 1e-6 relative L2, detector images
+<= 1e-4 relative per pixel).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests.golden import make_golden as mg
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load():
+    spec = dict(np.load(os.path.join(HERE, "allresul_specimen.npz")))
+    gold = dict(np.load(os.path.join(HERE, "allresul_golden.npz")))
+    return spec, gold
+
+
+@pytest.mark.parametrize("name", sorted(mg.CASES))
+def test_oracle_reproduces_golden(name):
+    spec, gold = _load()
+    assert spec["atoms"].shape == (791, 8) and len(spec["slicethick"]) == 41
+    cfg = mg.config(spec, name)
+    V, res, mid = mg.oracle_outputs(cfg)
+    got = mg.summarise(V, res, mid)
+    for k in ("myresul", "mid", "v_sum", "v_norm2"):
+        ref = gold["%s_%s" % (name, k)]
+        assert got[k].shape == ref.shape
+        np.testing.assert_allclose(got[k], ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+    np.testing.assert_array_equal(got["v_crop"], gold["%s_v_crop" % name])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(mg.CASES))
+def test_engine_matches_golden(engine, name):
+    import torch
+    from totem_amd import stem
+    assert torch.cuda.is_available()
+    print("case golden %s" % name, flush=True)
+    spec, gold = _load()
+    cfg = mg.config(spec, name)
+    inp = stem.prepare(cfg)
+    # detector images through the mex-shaped entry point
+    myresul, mid, _, _ = stem.run(inp)
+    for k, got in (("myresul", myresul), ("mid", mid)):
+        ref = gold["%s_%s" % (name, k)]
+        rel = np.abs(got - ref) / np.abs(ref)
+        print("  %s: max rel per pixel %.3e" % (k, rel.max()))
+        assert rel.max() <= 1e-4
+    # potential slices through the device-resident entry point
+    S, npx = cfg.n_slices, cfg.np_side
+    t = torch.empty((S, npx, npx), dtype=torch.complex64, device="cuda")
+    corr = cfg.paraflag == "n"
+    vre, vim = engine.build_transmission(
+        cfg.paraflag, npx, S, inp.series_n, inp.ele_n, inp.gx, inp.gy, inp.s2, inp.aper_true, inp.sigma,
+        inp.parameter, t.data_ptr(), series_n_corr=inp.series_n_corr if corr else None,
+        ele_n_corr=inp.ele_n_corr if corr else None, corr_info_matrix=inp.corr_matrix if corr else None,
+        want_v=True, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    V = vre + 1j * vim
+    crop = V[(slice(None),) + mg.CROP][list(mg.V_SLICES)]
+    ref = gold["%s_v_crop" % name].astype(np.complex128)
+    err = np.linalg.norm((crop - ref).ravel()) / np.linalg.norm(ref.ravel())
+    n2 = (np.abs(V) ** 2).sum(axis=(1, 2))
+    ref_n2 = gold["%s_v_norm2" % name]
+    live = ref_n2 > 0
+    print("  V crop rel L2 %.3e, max rel sum|V|^2 %.3e" % (err, np.max(np.abs(n2[live] / ref_n2[live] - 1))))
+    assert err <= 1e-6
+    assert np.all(np.abs(n2[live] / ref_n2[live] - 1) <= 2e-6)
+    assert np.all(n2[~live] == 0)

@@ -181,7 +181,7 @@ def test_stem_window_outside_potential_is_an_error(engine):
         stem.run(inp)
 
 
-def _device_job(engine, inp, t_dev, wrap=False, begin=None):
+def _device_job(engine, inp, t_dev, wrap=False, begin=None, exit_wave=None):
     torch = _torch()
     c = inp.cfg
     probes = torch.tensor(inp.probe.astype(np.complex64), device="cuda")
@@ -193,7 +193,8 @@ def _device_job(engine, inp, t_dev, wrap=False, begin=None):
     br, bc = (c.begin_row - 1, c.begin_col - 1) if begin is None else begin
     job = engine.stem_job(c.n, c.np_side, c.n_slices, t_dev.data_ptr(), probes.data_ptr(), M, prop.data_ptr(),
                           aper.data_ptr(), A, c.depth, c.width_red, c.step, br, bc, det.data_ptr(),
-                          aper2_ptr=aper2.data_ptr(), mid=sorted(c.mid_slices), wrap=wrap)
+                          aper2_ptr=aper2.data_ptr(), mid=sorted(c.mid_slices), wrap=wrap,
+                          exit_ptr=None if exit_wave is None else exit_wave.data_ptr())
     job._keep = (probes, prop, aper, aper2)
     return job, det
 
@@ -215,6 +216,68 @@ def test_batched_plain_and_graph_runs_are_bit_identical(engine):
     # and the device path agrees with the oracle fed the same (FP32-rounded) transmission
     res_o, mid_o = _oracle_stem(inp, t=to.astype(np.complex64).astype(np.complex128))
     _check_images(outs[0][1].T, res_o, "device API exit face")
+
+
+def _exit_wave_errors(inp, t_host, ew, det, begin, wrap):
+    """Exit-wave intensity rel L2 of every (member, position) and the detector errors against
+    the oracle fed the same FP32 transmission."""
+    c = inp.cfg
+    M, D = len(inp.gfsf), c.depth
+    ash = np.fft.ifftshift(inp.aper, axes=(-2, -1))
+    worst = 0.0
+    ref = np.zeros((D, inp.aper.shape[0]))
+    for d in range(D):
+        r0 = begin[0] + (d // c.width_red) * c.step
+        c0 = begin[1] + (d % c.width_red) * c.step
+        r, _, waves = oms.stem_position(t_host, inp.probe, inp.gfsf, inp.psf, ash, r0, c0, periodic=wrap, waves=True)
+        ref[d] = r
+        for m in range(M):
+            got = np.abs(ew[m * D + d].astype(np.complex128)) ** 2
+            exp = np.abs(waves[m]) ** 2
+            worst = max(worst, float(np.linalg.norm(got - exp) / np.linalg.norm(exp)))
+    return worst, ref
+
+
+def test_exit_wave_intensity_matches_oracle(engine):
+    torch = _torch()
+    print("case exit wave 256^2 x 24 slices, Peng+corr, M=2", flush=True)
+    cfg, inp = _stem_case("n", M=2, n=256, np_side=256, S=24, cells=4, scan=(2, 2), step=9, begin=(1, 1))
+    _, to = _oracle_potential(inp)
+    t32 = to.astype(np.complex64)
+    t_dev = torch.tensor(t32, device="cuda")
+    ew = torch.zeros((len(inp.gfsf) * cfg.depth, cfg.n, cfg.n), dtype=torch.complex64, device="cuda")
+    job, det = _device_job(engine, inp, t_dev, wrap=True, begin=(0, 0), exit_wave=ew)
+    engine.stem_run(job, 3, stream=_stream())
+    torch.cuda.synchronize()
+    worst, _ = _exit_wave_errors(inp, t32.astype(np.complex128), ew.cpu().numpy(), det, (0, 0), True)
+    print("  exit-wave intensity: max rel L2 %.3e (FP64-oracle error)" % worst)
+    assert worst <= 1e-5
+
+
+def test_exit_wave_full_size_phonon_configuration(engine):
+    """1024^2, 80 slices of a frozen-phonon SrTiO3 configuration (80 000 atoms, Peng + correction):
+    the exit wave of one probe against the FP64 oracle on the same transmission."""
+    torch = _torch()
+    print("case exit wave 1024^2 x 80 slices (phonon configuration)", flush=True)
+    cfg = stem.perovskite_c2(scan=1)
+    snap = stem.phonon_snapshots(cfg, 1, stem.C2_PHONON_SIGMA, seed=1002)[0]
+    rows = stem.prepare_specimen(cfg, snap)
+    inp = stem.prepare(cfg)
+    S, N = cfg.n_slices, cfg.n
+    t = torch.empty((S, N, N), dtype=torch.complex64, device="cuda")
+    engine.build_transmission("n", N, S, rows["series_n"], rows["ele_n"], inp.gx, inp.gy, inp.s2, inp.aper_true,
+                              inp.sigma, inp.parameter, t.data_ptr(), series_n_corr=rows["series_n_corr"],
+                              ele_n_corr=rows["ele_n_corr"], corr_info_matrix=rows["corr_matrix"], stream=_stream())
+    ew = torch.zeros((1, N, N), dtype=torch.complex64, device="cuda")
+    job, det = _device_job(engine, inp, t, wrap=True, begin=(500, 300), exit_wave=ew)
+    engine.stem_run(job, 1, stream=_stream())
+    torch.cuda.synchronize()
+    t_host = t.cpu().numpy().astype(np.complex128)
+    worst, ref = _exit_wave_errors(inp, t_host, ew.cpu().numpy(), det, (500, 300), True)
+    del t_host
+    print("  exit-wave intensity: rel L2 %.3e (FP64-oracle error)" % worst)
+    assert worst <= 1e-5
+    _check_images(det.cpu().numpy()[0].T, ref, "ADF/BF")
 
 
 def test_periodic_windows_equal_tiled_potential(engine):

@@ -31,73 +31,26 @@
 #include <vector>
 
 #include "../../include/totem_abi.h"
+#include "host_common.h"
 #include "kernels.h"
 #include "launch.h"
 #include "unit_round.h"
 
 namespace totem {
 
-// ----------------------------------------------------------------------- errors
-
-struct Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-static thread_local std::string g_last_error;
-
-#define HIP_OK(expr)                                                                                     \
-  do {                                                                                                   \
-    hipError_t e_ = (expr);                                                                              \
-    if (e_ != hipSuccess)                                                                                \
-      throw Error(e_ == hipErrorOutOfMemory ? TOTEM_ERR_MEMORY : TOTEM_ERR_DEVICE,                       \
-                  std::string(#expr) + " failed: " + hipGetErrorString(e_));                            \
-  } while (0)
-
-static void require(bool ok, const std::string& msg, int code = TOTEM_ERR_ARG) {
-  if (!ok) throw Error(code, msg);
+std::string& last_error() {
+  static thread_local std::string msg;
+  return msg;
 }
 
-template <class F> static int guarded(F&& f) {
-  try {
-    f();
-    g_last_error.clear();
-    return TOTEM_OK;
-  } catch (const Error& e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return TOTEM_ERR_DEVICE;
-  }
+// Grow-only device buffer: keeps its memory across calls, so steady-state calls
+// neither allocate nor free (hipFree would wait for all work on the device).
+static void reserve(DevBuf& b, size_t bytes) {
+  if (b.bytes < bytes) b.alloc(std::max<size_t>(bytes, 256));
 }
-
-// ----------------------------------------------------------------- device memory
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  explicit DevBuf(size_t b) { alloc(b); }
-  void alloc(size_t b) {
-    release();
-    if (b == 0) return;
-    HIP_OK(hipMalloc(&p, b));
-    bytes = b;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  ~DevBuf() { release(); }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  template <class T> T* as() const { return static_cast<T*>(p); }
-};
 
 template <class T> static void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
-  b.alloc(std::max<size_t>(v.size(), 1) * sizeof(T));
+  reserve(b, std::max<size_t>(v.size(), 1) * sizeof(T));
   if (!v.empty()) HIP_OK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
 }
 
@@ -350,7 +303,25 @@ static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev
     seg_off[s + 1] = (long)segs.size();
     max_terms = std::max(max_terms, v.size());
   }
-  DevBuf dtx, dty, dwt, dsegs, dcls, dkx, dky, ds2, dap, dcorr;
+  // per-device builder scratch, one build at a time per device
+  struct Scratch {
+    std::mutex mu;
+    DevBuf dtx, dty, dwt, dsegs, dcls, dkx, dky, ds2, dap, dcorr, dex, dey;
+  };
+  static std::mutex scratch_mu;
+  static std::map<int, std::unique_ptr<Scratch>> scratch;
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  Scratch* sc;
+  {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    auto& p = scratch[dev];
+    if (!p) p.reset(new Scratch());
+    sc = p.get();
+  }
+  std::lock_guard<std::mutex> build_lock(sc->mu);
+  DevBuf &dtx = sc->dtx, &dty = sc->dty, &dwt = sc->dwt, &dsegs = sc->dsegs, &dcls = sc->dcls, &dkx = sc->dkx,
+         &dky = sc->dky, &ds2 = sc->ds2, &dap = sc->dap, &dcorr = sc->dcorr;
   upload(dtx, tx, st);
   upload(dty, ty, st);
   upload(dwt, wt, st);
@@ -358,16 +329,17 @@ static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev
   upload(dcls, P.classes, st);
   upload(dkx, P.kx1, st);
   upload(dky, P.ky1, st);
-  ds2.alloc(np2 * sizeof(double));
+  reserve(ds2, np2 * sizeof(double));
   HIP_OK(hipMemcpyAsync(ds2.p, d.s_2, np2 * sizeof(double), hipMemcpyHostToDevice, st));
-  dap.alloc(np2 * sizeof(double));
+  reserve(dap, np2 * sizeof(double));
   HIP_OK(hipMemcpyAsync(dap.p, d.aper_true, np2 * sizeof(double), hipMemcpyHostToDevice, st));
   if (d.cim_len > 0) {
-    dcorr.alloc(np2 * d.cim_len * sizeof(double));
+    reserve(dcorr, np2 * d.cim_len * sizeof(double));
     HIP_OK(hipMemcpyAsync(dcorr.p, d.corr_info_matrix, np2 * d.cim_len * sizeof(double), hipMemcpyHostToDevice, st));
   }
-  DevBuf dex(std::max<size_t>(max_terms, 1) * np * sizeof(float2));
-  DevBuf dey(std::max<size_t>(max_terms, 1) * np * sizeof(float2));
+  DevBuf &dex = sc->dex, &dey = sc->dey;
+  reserve(dex, std::max<size_t>(max_terms, 1) * np * sizeof(float2));
+  reserve(dey, std::max<size_t>(max_terms, 1) * np * sizeof(float2));
   tw_table(np);
   const float sig = (float)(d.sigma / 1000.0);
   const bool want_v = v_re && v_im;
@@ -411,8 +383,6 @@ struct StemWorkspace {
   int* layer_hit;
 };
 
-static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
-
 static size_t stem_ws_bytes(int n, int S, int A, int L, int batch) {
   const int G = detector_groups(n);
   size_t b = 0;
@@ -443,7 +413,7 @@ static StemWorkspace carve(void* base, int n, int S, int A, int L, int batch) {
   return w;
 }
 
-static void validate_job(const totem_stem_job& j) {
+void validate_job(const totem_stem_job& j) {
   require(j.n > 0 && j.np > 0 && j.slices > 0, "grid sizes and slice count must be positive");
   require(j.n % 2 == 0, "probe window side must be even");
   require(size_supported(j.n), "probe grid " + std::to_string(j.n) + " has no compiled FFT plan",
@@ -463,6 +433,31 @@ static void validate_job(const totem_stem_job& j) {
             "probe window leaves the potential: last window ends at row " + std::to_string(rmax + j.n) +
                 ", column " + std::to_string(cmax + j.n) + " of " + std::to_string(j.np));
   }
+}
+
+void mid_walk(const totem_stem_job& j, std::vector<int>& mos, std::vector<int>& hit) {
+  // the reference advances one pointer through the sorted 1-based list
+  // (cuda_source_lobato.cu:1340-1385): unmatched or repeated entries are never written
+  mos.assign(j.slices, -1);
+  hit.assign(std::max(j.n_mid, 1), 0);
+  int li = 0;
+  for (int s = 0; s < j.slices; ++s)
+    if (li < j.n_mid && s + 1 == j.mid[li]) {
+      mos[s] = li;
+      hit[li] = 1;
+      ++li;
+    }
+}
+
+void window_origin(const totem_stem_job& j, int d, int& r0, int& c0) {
+  long r = (long)j.begin_row + (long)(d / j.width_red) * j.step;
+  long c = (long)j.begin_col + (long)(d % j.width_red) * j.step;
+  if (j.wrap) {
+    r %= j.np;
+    c %= j.np;
+  }
+  r0 = (int)r;
+  c0 = (int)c;
 }
 
 // Per-kernel durations of the last timed run (TOTEM_RUN_TIMING), from HIP events
@@ -490,15 +485,8 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   StemWorkspace w = carve(ws, n, S, A, L, batch);
   const int G = detector_groups(n);
   tw_table(n);
-  // mid-slice walk exactly as the reference: cuda_source_lobato.cu:1340
-  std::vector<int> mos(S, -1), hit(std::max(L, 1), 0);
-  int li = 0;
-  for (int s = 0; s < S; ++s)
-    if (li < L && s + 1 == j.mid[li]) {
-      mos[s] = li;
-      hit[li] = 1;
-      ++li;
-    }
+  std::vector<int> mos, hit;
+  mid_walk(j, mos, hit);
   JobDev jd;
   jd.t = static_cast<const float2*>(j.t);
   jd.prop = static_cast<const float2*>(j.prop);
@@ -507,6 +495,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   jd.mid_of_slice = L > 0 ? w.mid_of_slice : nullptr;
   jd.layer_hit = w.layer_hit;
   jd.det = j.det;
+  jd.exit = static_cast<float2*>(j.exit_wave);
   jd.np = j.np;
   jd.S = S;
   jd.A = A;
@@ -530,37 +519,31 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
       ItemDev& it = items[k];
       it.probe = probes + (size_t)m * n * n;
       it.job = 0;
-      long r0 = (long)j.begin_row + (long)(d / j.width_red) * j.step;
-      long c0 = (long)j.begin_col + (long)(d % j.width_red) * j.step;
-      if (j.wrap) {
-        r0 %= j.np;
-        c0 %= j.np;
-      }
-      it.r0 = (int)r0;
-      it.c0 = (int)c0;
+      window_origin(j, d, it.r0, it.c0);
       it.start_step = 0;
-      it.slot = k;
+      it.wave = w.psi + (size_t)k * n * n;
+      it.part = w.partial + (size_t)k * (L + 1) * A * G;
       it.out = (int)o;
     }
     HIP_OK(hipMemcpyAsync(w.items, items.data(), cnt * sizeof(ItemDev), hipMemcpyHostToDevice, st));
     auto record = [&]() {
       for (int s = 0; s < S; ++s) {
-        pass_rows(n, st, w.items, cnt, w.job, w.psi, s);
-        pass_cols(n, st, w.items, cnt, w.job, w.psi, s, w.partial, L + 1, A, s == S - 1 || mos[s] >= 0);
+        pass_rows(n, st, w.items, cnt, w.job, s);
+        pass_cols(n, st, w.items, cnt, w.job, s, s == S - 1 || mos[s] >= 0);
       }
-      reduce_det(st, w.items, nullptr, cnt, w.job, w.partial, G, L + 1, A);
+      reduce_det(st, w.items, nullptr, cnt, w.job, G, L + 1, A);
     };
     if (flags & TOTEM_RUN_TIMING) {
       std::vector<hipEvent_t> ev(2 * S + 1);
       for (auto& e : ev) HIP_OK(hipEventCreate(&e));
       HIP_OK(hipEventRecord(ev[0], st));
       for (int s = 0; s < S; ++s) {
-        pass_rows(n, st, w.items, cnt, w.job, w.psi, s);
+        pass_rows(n, st, w.items, cnt, w.job, s);
         HIP_OK(hipEventRecord(ev[2 * s + 1], st));
-        pass_cols(n, st, w.items, cnt, w.job, w.psi, s, w.partial, L + 1, A, s == S - 1 || mos[s] >= 0);
+        pass_cols(n, st, w.items, cnt, w.job, s, s == S - 1 || mos[s] >= 0);
         HIP_OK(hipEventRecord(ev[2 * s + 2], st));
       }
-      reduce_det(st, w.items, nullptr, cnt, w.job, w.partial, G, L + 1, A);
+      reduce_det(st, w.items, nullptr, cnt, w.job, G, L + 1, A);
       HIP_OK(hipEventSynchronize(ev[2 * S]));
       for (int s = 0; s < S; ++s) {
         float a = 0.f, b = 0.f;
@@ -792,8 +775,8 @@ int totem_potential_peng(TOTEM_MYPROJECTFUN_PARAMS) {
   return guarded([&] { project_impl(a, 'n', true); });
 }
 
-const char* totem_last_error(void) { return g_last_error.c_str(); }
-int totem_abi_version(void) { return 1; }
+const char* totem_last_error(void) { return totem::last_error().c_str(); }
+int totem_abi_version(void) { return 2; }
 int totem_fft_size_supported(int n) { return size_supported(n) ? 1 : 0; }
 
 int totem_build_transmission(const totem_potential_desc* desc, void* t_dev, double* v_re, double* v_im,

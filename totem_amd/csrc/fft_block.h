@@ -218,8 +218,40 @@ template <int R, int SIGN> __device__ __forceinline__ void dft(float2* a) {
 
 template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, int... RS> struct Stages;
 
+// Twiddles of the stage (P, R) for the U butterflies of lane t: block k = b mod P
+// of the stage table, R entries (index u*R + q; entry 0 is 1).  A stage's loads
+// are issued by the previous stage before its LDS exchange, so their latency
+// overlaps the exchange instead of stalling the butterflies after the barrier.
+template <int T, int P, int R, int U> struct StageTw {
+  static constexpr int NW = (P > 1) ? U * R : 1;
+  float2 w[NW];
+  __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
+    if constexpr (P > 1) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = (t + T * u) % P;
+        const float2* tk = tw + k * R;
+        if constexpr (R % 2 == 0) {
+          const float4* t4 = reinterpret_cast<const float4*>(__builtin_assume_aligned(tk, 16));
+#pragma unroll
+          for (int q = 0; q < R / 2; ++q) {
+            const float4 v = t4[q];
+            w[u * R + 2 * q] = make_float2(v.x, v.y);
+            w[u * R + 2 * q + 1] = make_float2(v.z, v.w);
+          }
+        } else {
+#pragma unroll
+          for (int q = 1; q < R; ++q) w[u * R + q] = tk[q];
+        }
+      }
+    }
+  }
+};
+
 template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING> struct Stages<N, T, SIGN, Lay, PP, P, PING> {
-  static __device__ __forceinline__ void run(float2*, float2*, float2*, const Lay&, int, const float2*) {}
+  using Tw = StageTw<T, 1, 1, 1>;
+  template <class W>
+  static __device__ __forceinline__ void run(float2*, float2*, float2*, const Lay&, int, const float2*, const W&) {}
 };
 
 template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, int R, int... RS>
@@ -229,10 +261,16 @@ struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
   static_assert(E % R == 0, "radix must divide elements per lane");
   static_assert(T % 16 == 0, "lane count must be a multiple of 16");
   static constexpr bool LAST = sizeof...(RS) == 0;
+  using Tw = StageTw<T, P, R, U>;
+  using Next = Stages<N, T, SIGN, Lay, PP, P * R, PP ? 1 - PING : 0, RS...>;
 
+  // `tw` points at this stage's table; `cur` holds this stage's twiddles.
   static __device__ __forceinline__ void run(float2* x, float2* lds0, float2* lds1, const Lay& lay, int t,
-                                             const float2* __restrict__ tw) {
+                                             const float2* __restrict__ tw, const Tw& cur) {
     float2* buf = PING ? lds1 : lds0;
+    const float2* tw_next = tw + (P > 1 ? P * R : 0);
+    typename Next::Tw nxt;
+    if constexpr (!LAST) nxt.load(tw_next, t);
     float2 out[E];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -242,24 +280,11 @@ struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
 #pragma unroll
       for (int q = 0; q < R; ++q) a[q] = x[u + q * U];
       if constexpr (P > 1) {
-        const float2* tk = tw + k * R;
-        float2 w[R];
-        if constexpr (R % 2 == 0) {
-          const float4* t4 = reinterpret_cast<const float4*>(__builtin_assume_aligned(tk, 16));
-#pragma unroll
-          for (int q = 0; q < R / 2; ++q) {
-            const float4 v = t4[q];
-            w[2 * q] = make_float2(v.x, v.y);
-            w[2 * q + 1] = make_float2(v.z, v.w);
-          }
-        } else {
-#pragma unroll
-          for (int q = 1; q < R; ++q) w[q] = tk[q];
-        }
 #pragma unroll
         for (int q = 1; q < R; ++q) {
-          if (SIGN > 0) w[q].y = -w[q].y;
-          a[q] = cmul(a[q], w[q]);
+          float2 w = cur.w[u * R + q];
+          if (SIGN > 0) w.y = -w.y;
+          a[q] = cmul(a[q], w);
         }
       }
       dft<R, SIGN>(a);
@@ -287,8 +312,7 @@ struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
 #pragma unroll
       for (int v = 0; v < E; ++v) x[v] = rp[Lay::off(T * v)];
       if constexpr (!PP) __syncthreads();
-      Stages<N, T, SIGN, Lay, PP, P * R, PP ? 1 - PING : 0, RS...>::run(x, lds0, lds1, lay, t,
-                                                                       tw + (P > 1 ? P * R : 0));
+      Next::run(x, lds0, lds1, lay, t, tw_next, nxt);
     }
   }
 };
@@ -331,7 +355,9 @@ template <int N> __host__ __device__ constexpr int twiddle_len() {
 template <int N, int SIGN, bool PP, int START = 0, class Lay>
 __device__ __forceinline__ void block_fft(float2* x, float2* lds0, float2* lds1, const Lay& lay, int t,
                                           const float2* tw) {
-  Plan<N>::template St<SIGN, Lay, PP, START>::run(x, lds0, lds1, lay, t, tw);
+  using St = typename Plan<N>::template St<SIGN, Lay, PP, START>;
+  typename St::Tw none;  // the first stage has no twiddles
+  St::run(x, lds0, lds1, lay, t, tw, none);
 }
 // START for a transform that follows one started on buffer `prev`.
 template <int N, bool PP> constexpr int next_start(int prev) { return PP ? (prev + Plan<N>::EXCH) % 2 : 0; }
@@ -347,12 +373,16 @@ struct RowLay {
   static __device__ __forceinline__ constexpr int off(int k) { return k + (k >> 4); }
   template <int P, int R> static constexpr bool linear_write() { return P % 16 == 0 || (P == 1 && R % 16 == 0); }
 };
+// Column transforms also pad one slot-row per 16 elements: without it the first
+// stage's stride-R writes of lanes t, t+1, ... land R*C slots apart, i.e. on the
+// same banks.
 template <int C> struct ColLay {
   int c;
-  __device__ __forceinline__ int operator()(int e) const { return e * C + c; }
-  static __device__ __forceinline__ constexpr int off(int k) { return k * C; }
-  template <int P, int R> static constexpr bool linear_write() { return true; }
+  __device__ __forceinline__ int operator()(int e) const { return (e + (e >> 4)) * C + c; }
+  static __device__ __forceinline__ constexpr int off(int k) { return (k + (k >> 4)) * C; }
+  template <int P, int R> static constexpr bool linear_write() { return P % 16 == 0 || (P == 1 && R % 16 == 0); }
 };
 template <int N> __host__ __device__ constexpr int row_lds_len() { return N + N / 16; }
+template <int N, int C> __host__ __device__ constexpr int col_lds_len() { return (N + N / 16) * C; }
 
 }  // namespace totem

@@ -20,18 +20,21 @@ struct JobDev {
   int S;                      // slices
   int A;                      // detectors
   int L;                      // mid layers
+  float2* exit;               // exit-face wave Psi(k) [n_out][N][N] (zero frequency at [0][0]) or null
   int n_out;                  // output columns (probe positions x focal members)
   int wrap;                   // periodic window into the potential
 };
 
-// One probe position of one job (a "work item"); occupies wave slot `slot`.
+// One probe position of one job (a "work item") and the device memory it owns
+// while in flight: a wave buffer and its detector partials.
 struct ItemDev {
   const float2* probe;        // initial wave [N][N], complex64
+  float2* wave;               // wave buffer [N][N]
+  double* part;               // detector partials [lmax1][amax][G] (G = column groups of the N grid)
   int job;                    // index into the job table
   int r0, c0;                 // window origin (row, column) in the potential
-  int start_step;             // global step at which the item's slice 0 runs
-  int slot;                   // wave buffer slot
-  int out;                    // output column in job.det
+  int start_step;             // launch step at which the item's slice 0 runs
+  int out;                    // output column in job.det / job.exit
 };
 
 // Form-factor class of the potential builder.

@@ -40,28 +40,12 @@ namespace totem {
 
 // ------------------------------------------------------------------------- helpers
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// With the remap, XCD x works through the groups [x*G/8, (x+1)*G/8) of every probe
-// in order: neighbouring column groups (which share 128-byte lines) run back to back
-// on one L2, and each L2 only ever sees 1/8 of the propagator / a band of the
-// transmission slice.  Any G divisible by 8 gives a bijection; otherwise the plain
-// probe-major order is used.
-__device__ __forceinline__ void xcd_map(int bid, int G, int& item, int& g) {
-#if TOTEM_XCD_REMAP
-  if ((G & 7) == 0) {
-    const int xcd = bid & 7, k = bid >> 3, gp = G >> 3;
-    item = k / gp;
-    g = xcd * gp + (k - item * gp);
-    return;
-  }
+#ifndef TOTEM_ROWS_THREADS
+#define TOTEM_ROWS_THREADS 256  // threads per rows-pass workgroup
 #endif
-  item = bid / G;
-  g = bid - item * G;
-}
-
 template <int N> struct RowCfg {
   static constexpr int T = Plan<N>::T;
-  static constexpr int RW = (256 / T) > 0 ? 256 / T : 1;  // rows per workgroup
+  static constexpr int RW = (TOTEM_ROWS_THREADS / T) > 0 ? TOTEM_ROWS_THREADS / T : 1;  // rows per workgroup
   static constexpr int THREADS = RW * T;
   static constexpr int LL = row_lds_len<N>();
   static constexpr bool PP = TOTEM_ROWS_PP != 0;
@@ -71,9 +55,18 @@ template <int N> struct ColCfg {
   static constexpr bool PP = TOTEM_COLS_PP != 0;
   static constexpr int NBUF = PP ? 2 : 1;
   static constexpr int T = Plan<N>::T;
-  static constexpr int C = (N <= 512) ? 16 : (N >= 1536 ? 4 : (N == 1024 ? TOTEM_COLS_C1024 : 8));  // columns per workgroup
+  // columns per workgroup
+  static constexpr int C = (N <= 512) ? 16 : (N >= 1536 ? 4 : (N == 1024 ? TOTEM_COLS_C1024 : 8));
   static constexpr int THREADS = T * C;
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// With the remap (unit_map below), XCD x works through the groups
+// [x*G/8, (x+1)*G/8) of every probe in order: neighbouring column groups (which
+// share 128-byte lines) run back to back on one L2, and each L2 only ever sees 1/8
+// of the propagator / a band of the transmission slice.  Any G with G/8 divisible
+// by the units per workgroup gives a bijection; otherwise the plain probe-major
+// order is used.
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -83,51 +76,131 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // ------------------------------------------------------------------------ pass R
 
-template <int N>
-__global__ void __launch_bounds__(RowCfg<N>::THREADS, TOTEM_ROWS_MINWAVES)
-k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, float2* __restrict__ psi,
-            int step, const float2* __restrict__ tw) {
+#ifndef TOTEM_ROWS_TPREFETCH
+#define TOTEM_ROWS_TPREFETCH 1  // load the transmission window together with the wave
+#endif
+#ifndef TOTEM_ROWS_K
+#define TOTEM_ROWS_K 1  // row groups per workgroup (loads of the next one overlap the FFTs of this one)
+#endif
+#ifndef TOTEM_COLS_HPREFETCH
+#define TOTEM_COLS_HPREFETCH 1  // load the propagator before the forward transform
+#endif
+#ifndef TOTEM_COLS_K
+#define TOTEM_COLS_K 1
+#endif
+
+// Unit u (< K) of workgroup `bid` -> (probe item, group g of G per probe).
+template <int K>
+__device__ __forceinline__ void unit_map(int bid, int u, int G, int& item, int& g) {
+#if TOTEM_XCD_REMAP
+  if ((G & 7) == 0 && ((G >> 3) % K) == 0) {
+    const int xcd = bid & 7, gp = G >> 3;
+    const int q = (bid >> 3) * K + u;
+    item = q / gp;
+    g = xcd * gp + (q - item * gp);
+    return;
+  }
+#endif
+  const int L = bid * K + u;
+  item = L / G;
+  g = L - item * G;
+}
+
+struct RowUnit {
+  bool active;
+  int slice;
+  const float2* src;   // wave read: probe (slice 0) or psi
+  float2* dst;         // psi row
+  const float2* trow;  // transmission row
+  int c0, np;
+};
+
+template <int N, int K>
+__device__ __forceinline__ RowUnit row_unit(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs,
+                                            int step, int u, int lr, int t) {
   using Cfg = RowCfg<N>;
-  constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, G = N / RW;
-  __shared__ float2 lds[Cfg::NBUF * RW * Cfg::LL];
-  float2* const lds1 = lds + (Cfg::NBUF - 1) * RW * Cfg::LL;
+  constexpr int RW = Cfg::RW, G = N / RW;
+  RowUnit r;
   int item, rg;
-  xcd_map(blockIdx.x, G, item, rg);
+  unit_map<K>(blockIdx.x, u, G, item, rg);
   const ItemDev it = items[item];
   const JobDev& jb = jobs[it.job];
-  const int slice = step - it.start_step;
-  if (slice < 0 || slice >= jb.S) return;  // uniform over the workgroup
+  r.slice = step - it.start_step;
+  r.active = r.slice >= 0 && r.slice < jb.S;
+  const int row = rg * RW + lr;
+  r.dst = it.wave + (size_t)row * N + t;
+  r.src = r.slice == 0 ? it.probe + (size_t)row * N + t : r.dst;
+  r.np = jb.np;
+  int pr = it.r0 + row;
+  if (pr >= r.np) pr -= r.np;  // only reachable with wrap (validated on the host)
+  r.trow = jb.t + (size_t)(r.active ? r.slice : 0) * r.np * r.np + (size_t)pr * r.np;
+  r.c0 = it.c0;
+  return r;
+}
+
+template <int N>
+__global__ void __launch_bounds__(RowCfg<N>::THREADS, TOTEM_ROWS_MINWAVES)
+k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step, const float2* __restrict__ tw) {
+  using Cfg = RowCfg<N>;
+  constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, K = TOTEM_ROWS_K;
+  __shared__ float2 lds[Cfg::NBUF * RW * Cfg::LL];
+  float2* const lds1 = lds + (Cfg::NBUF - 1) * RW * Cfg::LL;
   const int lr = threadIdx.x / T;
   const int t = threadIdx.x - lr * T;
-  const int row = rg * RW + lr;
   const RowLay lay{lr * Cfg::LL};
-  float2 x[E];
-  float2* w = psi + (size_t)it.slot * N * N + (size_t)row * N + t;
-  if (slice == 0) {
-    const float2* p = it.probe + (size_t)row * N + t;
+  float2 xa[E], xb[E];
+  RowUnit ua = row_unit<N, K>(items, jobs, step, 0, lr, t);
+  if (ua.active) {
 #pragma unroll
-    for (int v = 0; v < E; ++v) x[v] = gld(p + T * v);
-  } else {
-#pragma unroll
-    for (int v = 0; v < E; ++v) x[v] = gld(w + T * v);
-    block_fft<N, +1, Cfg::PP, 0>(x, lds, lds1, lay, t, tw);
-    const float s = 1.0f / (float)(N * N);
-#pragma unroll
-    for (int v = 0; v < E; ++v) x[v] = cscale(x[v], s);
+    for (int v = 0; v < E; ++v) xa[v] = gld(ua.src + T * v);
   }
-  const int np = jb.np;
-  int pr = it.r0 + row;
-  if (pr >= np) pr -= np;  // only reachable with wrap (validated on the host)
-  const float2* trow = jb.t + (size_t)slice * np * np + (size_t)pr * np;
 #pragma unroll
-  for (int v = 0; v < E; ++v) {
-    int pc = it.c0 + t + T * v;
-    if (pc >= np) pc -= np;
-    x[v] = cmul(x[v], gld(trow + pc));
+  for (int u = 0; u < K; ++u) {
+    RowUnit ub;
+    if (u + 1 < K) {
+      ub = row_unit<N, K>(items, jobs, step, u + 1, lr, t);
+      if (ub.active) {
+#pragma unroll
+        for (int v = 0; v < E; ++v) xb[v] = gld(ub.src + T * v);
+      }
+    }
+    if (ua.active) {  // uniform over the workgroup
+      float2 tv[TOTEM_ROWS_TPREFETCH ? E : 1];
+      if constexpr (TOTEM_ROWS_TPREFETCH) {
+#pragma unroll
+        for (int v = 0; v < E; ++v) {
+          int pc = ua.c0 + t + T * v;
+          if (pc >= ua.np) pc -= ua.np;
+          tv[v] = gld(ua.trow + pc);
+        }
+      }
+      if (ua.slice > 0) {
+        block_fft<N, +1, Cfg::PP, 0>(xa, lds, lds1, lay, t, tw);
+        const float s = 1.0f / (float)(N * N);
+#pragma unroll
+        for (int v = 0; v < E; ++v) xa[v] = cscale(xa[v], s);
+      }
+#pragma unroll
+      for (int v = 0; v < E; ++v) {
+        if constexpr (TOTEM_ROWS_TPREFETCH) {
+          xa[v] = cmul(xa[v], tv[v]);
+        } else {
+          int pc = ua.c0 + t + T * v;
+          if (pc >= ua.np) pc -= ua.np;
+          xa[v] = cmul(xa[v], gld(ua.trow + pc));
+        }
+      }
+      block_fft<N, -1, Cfg::PP, next_start<N, Cfg::PP>(0)>(xa, lds, lds1, lay, t, tw);
+#pragma unroll
+      for (int v = 0; v < E; ++v) gst(ua.dst + T * v, xa[v]);
+      if (u + 1 < K) __syncthreads();  // LDS reuse by the next unit
+    }
+    if (u + 1 < K) {
+      ua = ub;
+#pragma unroll
+      for (int v = 0; v < E; ++v) xa[v] = xb[v];
+    }
   }
-  block_fft<N, -1, Cfg::PP, next_start<N, Cfg::PP>(0)>(x, lds, lds1, lay, t, tw);
-#pragma unroll
-  for (int v = 0; v < E; ++v) gst(w + T * v, x[v]);
 }
 
 // ------------------------------------------------------------------------ pass C
@@ -168,62 +241,120 @@ __device__ __forceinline__ void detector_partial(const float2* x, const double* 
   __syncthreads();
 }
 
+struct ColUnit {
+  bool active, last;
+  int slice, mid, cg, out;
+  __amdgpu_buffer_rsrc_t wr;
+  const JobDev* jb;
+  double* pbase;
+};
+
+template <int N, int K>
+__device__ __forceinline__ ColUnit col_unit(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs,
+                                            int step, int u, bool det) {
+  using Cfg = ColCfg<N>;
+  constexpr int G = N / Cfg::C;
+  ColUnit r;
+  int item;
+  unit_map<K>(blockIdx.x, u, G, item, r.cg);
+  const ItemDev it = items[item];
+  r.jb = &jobs[it.job];
+  r.slice = step - it.start_step;
+  r.active = r.slice >= 0 && r.slice < r.jb->S;
+  r.wr = buf_rsrc(it.wave, (unsigned)N * N * sizeof(float2));
+  r.last = r.slice == r.jb->S - 1;
+  r.mid = (det && r.active && r.jb->mid_of_slice) ? r.jb->mid_of_slice[r.slice] : -1;
+  r.pbase = it.part;
+  r.out = it.out;
+  return r;
+}
+
 // DET = false: no item of the launch is at a detector slice (the lean variant).
 template <int N, bool DET>
 __global__ void __launch_bounds__(ColCfg<N>::THREADS, TOTEM_COLS_MINWAVES)
-k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, float2* __restrict__ psi,
-            int step, const float2* __restrict__ tw, double* __restrict__ partial, int lmax1, int amax) {
+k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step,
+            const float2* __restrict__ tw) {
   using Cfg = ColCfg<N>;
-  constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C;
+  constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C, K = TOTEM_COLS_K;
   constexpr int NW = (Cfg::THREADS + 63) / 64;
-  __shared__ float2 lds[Cfg::NBUF * N * C];
+  constexpr int LC = col_lds_len<N, C>();
+  __shared__ float2 lds[Cfg::NBUF * LC];
   __shared__ double red[DET ? NW * kMaxAper : 1];
-  float2* const lds1 = lds + (Cfg::NBUF - 1) * N * C;
-  int item, cg;
-  xcd_map(blockIdx.x, G, item, cg);
-  const ItemDev it = items[item];
-  const JobDev& jb = jobs[it.job];
-  const int slice = step - it.start_step;
-  if (slice < 0 || slice >= jb.S) return;
+  float2* const lds1 = lds + (Cfg::NBUF - 1) * LC;
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
-  const int col = cg * C + c;
   const ColLay<C> lay{c};
-  float2 x[E];
   constexpr unsigned kBytes = (unsigned)N * N * sizeof(float2);
-  const auto wr = buf_rsrc(psi + (size_t)it.slot * N * N, kBytes);
-  const int voff = (t * N + col) * (int)sizeof(float2);
   constexpr int kStep = T * N * (int)sizeof(float2);
+  float2 xa[E], xb[E];
+  ColUnit ua = col_unit<N, K>(items, jobs, step, 0, DET);
+  int voffa = (t * N + ua.cg * C + c) * (int)sizeof(float2);
+  if (ua.active) {
 #pragma unroll
-  for (int v = 0; v < E; ++v) x[v] = bld(wr, voff, v * kStep);
-  block_fft<N, -1, Cfg::PP, 0>(x, lds, lds1, lay, t, tw);
-  const auto hr = buf_rsrc(jb.prop, kBytes);
+    for (int v = 0; v < E; ++v) xa[v] = bld(ua.wr, voffa, v * kStep);
+  }
 #pragma unroll
-  for (int v = 0; v < E; ++v) x[v] = cmul(x[v], bld(hr, voff, v * kStep));
-  if constexpr (DET) {
-    const int mid = jb.mid_of_slice ? jb.mid_of_slice[slice] : -1;
-    double* pbase = partial + (size_t)it.slot * lmax1 * amax * G;
-    if (mid >= 0) {
-      for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
-        detector_partial<N, E, T, C>(x, jb.aper2 + (size_t)a0 * N * N, false, min(kMaxAper, jb.A - a0), t, col,
-                                     red, pbase + ((size_t)mid * amax + a0) * G, G, cg);
+  for (int u = 0; u < K; ++u) {
+    ColUnit ub;
+    int voffb = 0;
+    if (u + 1 < K) {
+      ub = col_unit<N, K>(items, jobs, step, u + 1, DET);
+      voffb = (t * N + ub.cg * C + c) * (int)sizeof(float2);
+      if (ub.active) {
+#pragma unroll
+        for (int v = 0; v < E; ++v) xb[v] = bld(ub.wr, voffb, v * kStep);
+      }
     }
-    if (slice == jb.S - 1) {
-      for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
-        detector_partial<N, E, T, C>(x, jb.aper + (size_t)a0 * N * N, true, min(kMaxAper, jb.A - a0), t, col, red,
-                                     pbase + ((size_t)jb.L * amax + a0) * G, G, cg);
-      return;
+    if (ua.active) {  // uniform over the workgroup
+      const JobDev& jb = *ua.jb;
+      const int col = ua.cg * C + c;
+      const auto hr = buf_rsrc(jb.prop, kBytes);
+      float2 hv[TOTEM_COLS_HPREFETCH ? E : 1];
+      if constexpr (TOTEM_COLS_HPREFETCH) {
+#pragma unroll
+        for (int v = 0; v < E; ++v) hv[v] = bld(hr, voffa, v * kStep);
+      }
+      block_fft<N, -1, Cfg::PP, 0>(xa, lds, lds1, lay, t, tw);
+#pragma unroll
+      for (int v = 0; v < E; ++v) xa[v] = cmul(xa[v], TOTEM_COLS_HPREFETCH ? hv[v] : bld(hr, voffa, v * kStep));
+      bool keep = true;
+      if constexpr (DET) {
+        if (ua.mid >= 0) {
+          for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
+            detector_partial<N, E, T, C>(xa, jb.aper2 + (size_t)a0 * N * N, false, min(kMaxAper, jb.A - a0), t,
+                                         col, red, ua.pbase + ((size_t)ua.mid * jb.A + a0) * G, G, ua.cg);
+        }
+        if (ua.last) {
+          for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
+            detector_partial<N, E, T, C>(xa, jb.aper + (size_t)a0 * N * N, true, min(kMaxAper, jb.A - a0), t, col,
+                                         red, ua.pbase + ((size_t)jb.L * jb.A + a0) * G, G, ua.cg);
+          keep = false;  // the exit face needs no inverse transform
+          if (jb.exit) {
+            const auto er = buf_rsrc(jb.exit + (size_t)ua.out * N * N, kBytes);
+#pragma unroll
+            for (int v = 0; v < E; ++v) bst(er, voffa, v * kStep, xa[v]);
+          }
+        }
+      }
+      if (keep) {
+        block_fft<N, +1, Cfg::PP, next_start<N, Cfg::PP>(0)>(xa, lds, lds1, lay, t, tw);
+#pragma unroll
+        for (int v = 0; v < E; ++v) bst(ua.wr, voffa, v * kStep, xa[v]);
+      }
+      if (u + 1 < K) __syncthreads();  // LDS reuse by the next unit
+    }
+    if (u + 1 < K) {
+      ua = ub;
+      voffa = voffb;
+#pragma unroll
+      for (int v = 0; v < E; ++v) xa[v] = xb[v];
     }
   }
-  block_fft<N, +1, Cfg::PP, next_start<N, Cfg::PP>(0)>(x, lds, lds1, lay, t, tw);
-#pragma unroll
-  for (int v = 0; v < E; ++v) bst(wr, voff, v * kStep, x[v]);
 }
 
 // Fixed-order Kahan sum of the detector partials of the listed items.
 __global__ void k_reduce_det(const ItemDev* __restrict__ items, const int* __restrict__ list, int n,
-                             const JobDev* __restrict__ jobs, const double* __restrict__ partial, int G,
-                             int lmax1, int amax) {
+                             const JobDev* __restrict__ jobs, int G, int lmax1, int amax) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n * lmax1 * amax) return;
   const int a = idx % amax;
@@ -234,7 +365,7 @@ __global__ void k_reduce_det(const ItemDev* __restrict__ items, const int* __res
   if (a >= jb.A || layer > jb.L) return;
   double s = 0.0;
   if (layer == jb.L || jb.layer_hit[layer]) {
-    const double* p = partial + (((size_t)it.slot * lmax1 + layer) * amax + a) * G;
+    const double* p = it.part + ((size_t)layer * jb.A + a) * G;
     double comp = 0.0;
     for (int g = 0; g < G; ++g) {
       const double y = p[g] - comp;
@@ -275,7 +406,7 @@ k_fft_cols(const float2* __restrict__ src, float2* __restrict__ dst, int epi, fl
            const float2* __restrict__ tw) {
   using Cfg = ColCfg<N>;
   constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C;
-  __shared__ float2 lds[N * C];
+  __shared__ float2 lds[col_lds_len<N, C>()];
   const int mat = blockIdx.x / G;
   const int cg = blockIdx.x - mat * G;
   const int c = threadIdx.x % C;
@@ -418,21 +549,23 @@ k_accumulate(const float2* __restrict__ ex, const float2* __restrict__ ey, const
 // --------------------------------------------------------------------- launchers
 
 template <int N> void launch_pass_rows(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
-                                       float2* psi, int step) {
+                                       int step) {
   using Cfg = RowCfg<N>;
-  const dim3 grid((unsigned)(n_items * (N / Cfg::RW)));
-  hipLaunchKernelGGL(k_pass_rows<N>, grid, dim3(Cfg::THREADS), 0, st, items, jobs, psi, step, tw_table(N));
+  constexpr int K = TOTEM_ROWS_K;
+  static_assert((N / Cfg::RW) % K == 0, "row groups per probe must be a multiple of K");
+  const dim3 grid((unsigned)(n_items * (N / Cfg::RW) / K));
+  hipLaunchKernelGGL(k_pass_rows<N>, grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, tw_table(N));
 }
 template <int N> void launch_pass_cols(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
-                                       float2* psi, int step, double* partial, int lmax1, int amax, bool det) {
+                                       int step, bool det) {
   using Cfg = ColCfg<N>;
-  const dim3 grid((unsigned)(n_items * (N / Cfg::C)));
+  constexpr int K = TOTEM_COLS_K;
+  static_assert((N / Cfg::C) % K == 0, "column groups per probe must be a multiple of K");
+  const dim3 grid((unsigned)(n_items * (N / Cfg::C) / K));
   if (det)
-    hipLaunchKernelGGL((k_pass_cols<N, true>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, psi, step, tw_table(N),
-                       partial, lmax1, amax);
+    hipLaunchKernelGGL((k_pass_cols<N, true>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, tw_table(N));
   else
-    hipLaunchKernelGGL((k_pass_cols<N, false>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, psi, step,
-                       tw_table(N), partial, lmax1, amax);
+    hipLaunchKernelGGL((k_pass_cols<N, false>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, tw_table(N));
 }
 template <int N> void launch_fft2d(hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi,
                                    float sigma) {
@@ -487,25 +620,24 @@ int detector_groups(int n) {
   return 0;
 }
 
-void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step) {
-#define X(S) if (n == S) { launch_pass_rows<S>(st, items, n_items, jobs, psi, step); return; }
+void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step) {
+#define X(S) if (n == S) { launch_pass_rows<S>(st, items, n_items, jobs, step); return; }
   TOTEM_FOR_SIZES(X)
 #undef X
 }
 
-void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step,
-               double* partial, int lmax1, int amax, bool det) {
-#define X(S) if (n == S) { launch_pass_cols<S>(st, items, n_items, jobs, psi, step, partial, lmax1, amax, det); return; }
+void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step, bool det) {
+#define X(S) if (n == S) { launch_pass_cols<S>(st, items, n_items, jobs, step, det); return; }
   TOTEM_FOR_SIZES(X)
 #undef X
 }
 
-void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs,
-                const double* partial, int groups, int lmax1, int amax) {
+void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs, int groups,
+                int lmax1, int amax) {
   const int total = n * lmax1 * amax;
   if (total <= 0) return;
-  hipLaunchKernelGGL(k_reduce_det, dim3((total + 255) / 256), dim3(256), 0, st, items, list, n, jobs, partial,
-                     groups, lmax1, amax);
+  hipLaunchKernelGGL(k_reduce_det, dim3((total + 255) / 256), dim3(256), 0, st, items, list, n, jobs, groups, lmax1,
+                     amax);
 }
 
 void fft2d(int n, hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi, float sigma) {

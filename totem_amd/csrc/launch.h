@@ -17,13 +17,16 @@ void twiddle_host(int n, std::vector<float2>& out);
 bool size_supported(int n);   // FFT lengths with a compiled radix plan
 int detector_groups(int n);   // column groups per probe in k_pass_cols
 
-void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step);
+// One slice step of every listed item (slice = step - item.start_step; items outside
+// [0, S) of their job are skipped).
+void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
 // det: some item of the launch may be at a detector slice (mid-slice or exit face);
 // false selects the variant without detector code.
-void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, float2* psi, int step,
-               double* partial, int lmax1, int amax, bool det);
-void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs,
-                const double* partial, int groups, int lmax1, int amax);
+void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step, bool det);
+// Detector sums of the items items[list[e]] (list may be null: items[e]), e < n, into their
+// job's det; lmax1 and amax bound (L + 1) and A over the jobs involved.
+void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs, int groups,
+                int lmax1, int amax);
 // In-place batched 2-D FFT of `batch` n x n complex64 matrices; the column pass
 // writes to dst (may equal data).  epi: 0 plain, 1 transmission exp(i sigma V), 2 scale 1/n^2.
 void fft2d(int n, hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi, float sigma);

@@ -35,10 +35,13 @@ EXPORTS = (
     "totem_project_lobato", "totem_project_peng", "totem_potential_lobato", "totem_potential_peng",
     "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
     "totem_stem_workspace_bytes", "totem_stem_run", "totem_kernel_times", "totem_fft2d",
+    "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
+    "totem_server_destroy",
 )
 
 TOTEM_RUN_GRAPH = 1
 TOTEM_RUN_TIMING = 2
+ABI_VERSION = 2   # layout of the structs below (totem_abi_version())
 
 
 class TotemError(RuntimeError):
@@ -76,7 +79,7 @@ class StemJob(ctypes.Structure):
         ("aper", _VP), ("aper2", _VP), ("n_aper", _I),
         ("mid", _IP), ("n_mid", _I),
         ("depth", _I), ("width_red", _I), ("step", _I), ("begin_row", _I), ("begin_col", _I), ("wrap", _I),
-        ("det", _VP),
+        ("det", _VP), ("exit_wave", _VP),
     ]
 
 
@@ -113,6 +116,17 @@ def lib():
         L.totem_kernel_times.restype = None
         L.totem_fft2d.argtypes = [_VP, _I, _I, _I, _VP]
         L.totem_fft2d.restype = _I
+        L.totem_server_create.argtypes = [ctypes.c_size_t, _I, _I, ctypes.POINTER(_VP)]
+        L.totem_server_submit.argtypes = [_VP, ctypes.POINTER(StemJob), ctypes.POINTER(ctypes.c_longlong)]
+        L.totem_server_step.argtypes = [_VP, ctypes.POINTER(_I)]
+        L.totem_server_poll.argtypes = [_VP, ctypes.POINTER(ctypes.c_longlong), _I, ctypes.POINTER(_I)]
+        L.totem_server_sync.argtypes = [_VP]
+        L.totem_server_destroy.argtypes = [_VP]
+        for name in ("create", "submit", "step", "poll", "sync", "destroy"):
+            getattr(L, "totem_server_" + name).restype = _I
+        if L.totem_abi_version() != ABI_VERSION:
+            raise RuntimeError("%s has ABI version %d, this binding expects %d (rebuild)"
+                               % (CORE, L.totem_abi_version(), ABI_VERSION))
         _lib = L
     return _lib
 
@@ -245,7 +259,7 @@ def build_transmission(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx,
 
 
 def stem_job(n, np_side, slices, t_ptr, probes_ptr, n_probes, prop_ptr, aper_ptr, n_aper, depth, width_red,
-             step, begin_row, begin_col, det_ptr, aper2_ptr=None, mid=(), wrap=False):
+             step, begin_row, begin_col, det_ptr, aper2_ptr=None, mid=(), wrap=False, exit_ptr=None):
     """Fill a StemJob (device pointers are integers, e.g. torch ``tensor.data_ptr()``)."""
     j = StemJob()
     j.n, j.np, j.slices = int(n), int(np_side), int(slices)
@@ -258,6 +272,7 @@ def stem_job(n, np_side, slices, t_ptr, probes_ptr, n_probes, prop_ptr, aper_ptr
     j.depth, j.width_red, j.step = int(depth), int(width_red), int(step)
     j.begin_row, j.begin_col, j.wrap = int(begin_row), int(begin_col), int(bool(wrap))
     j.det = det_ptr
+    j.exit_wave = exit_ptr
     return j
 
 
@@ -285,3 +300,61 @@ def fft2d(ptr, n, batch, sign, stream=None):
 
 def fft_size_supported(n):
     return bool(lib().totem_fft_size_supported(int(n)))
+
+
+# ------------------------------------------------------------------ served engine
+
+
+class Server:
+    """Continuous batching of device STEM jobs (``totem_server_*``): submit jobs at any time,
+    call :meth:`step` to advance every in-flight probe by one slice, :meth:`poll` for the ids
+    of finished jobs.  A job's device arrays must stay alive until it is reported finished."""
+
+    def __init__(self, arena_bytes, max_aper=8, max_mid=16):
+        h = _VP()
+        check(lib().totem_server_create(ctypes.c_size_t(int(arena_bytes)), int(max_aper), int(max_mid),
+                                        ctypes.byref(h)))
+        self._h = h
+        self._jobs = {}
+        self._ids = (ctypes.c_longlong * 256)()
+
+    def submit(self, job):
+        jid = ctypes.c_longlong()
+        check(lib().totem_server_submit(self._h, ctypes.byref(job), ctypes.byref(jid)))
+        self._jobs[jid.value] = job   # keeps the job's host-side arrays (mid list) alive
+        return jid.value
+
+    def step(self):
+        n = _I()
+        check(lib().totem_server_step(self._h, ctypes.byref(n)))
+        return n.value
+
+    def poll(self):
+        out = []
+        while True:
+            n = _I()
+            check(lib().totem_server_poll(self._h, self._ids, len(self._ids), ctypes.byref(n)))
+            out.extend(self._ids[k] for k in range(n.value))
+            if n.value < len(self._ids):
+                break
+        for jid in out:
+            self._jobs.pop(jid, None)
+        return out
+
+    @property
+    def busy(self):
+        return bool(self._jobs)
+
+    def sync(self):
+        check(lib().totem_server_sync(self._h))
+
+    def close(self):
+        if self._h:
+            check(lib().totem_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

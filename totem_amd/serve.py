@@ -1,0 +1,184 @@
+"""Serving: STEM requests of mixed grid sizes on one GPU with continuous batching.
+
+A request is one STEM simulation (specimen, optics, scan).  The loop
+(:func:`serve`) admits arrived requests while their device footprint fits the
+HBM budget, builds each admitted request's transmission stack on the GPU,
+submits its probes to the engine's served form (:class:`engine.Server`, which
+advances probes of all in-flight jobs together, one slice per step, and admits
+new probes as wave slots free up), and records per-request latency from arrival
+to results on the host.
+
+This is the request loop around the reference's one-call-per-simulation
+``myProjectfun`` (``cuda_source_lobato.cu:1018-1562``): the arithmetic of each
+job is the same as a call of its own.
+"""
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import engine, optics, specimen, stem
+
+GiB = 1 << 30
+
+
+class DeviceJob:
+    """Device-resident inputs and outputs of one STEM job (torch tensors on cuda)."""
+
+    def __init__(self, inp, dev="cuda", stream=None, wrap=None, exit_wave=False):
+        import torch
+        c = inp.cfg
+        self.inp = inp
+        self.S, self.N, self.NP = c.n_slices, c.n, c.np_side
+        self.M, self.A, self.D, self.L = len(inp.gfsf), inp.aper.shape[0], c.depth, len(c.mid_slices)
+        wrap = (c.n == c.np_side) if wrap is None else wrap
+        st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+        self.t = torch.empty((self.S, self.NP, self.NP), dtype=torch.complex64, device=dev)
+        corr = c.paraflag == "n" and inp.corr_matrix is not None and len(inp.corr_matrix) > 0
+        engine.build_transmission(
+            c.paraflag, self.NP, self.S, inp.series_n, inp.ele_n, inp.gx, inp.gy, inp.s2, inp.aper_true,
+            inp.sigma, inp.parameter, self.t.data_ptr(),
+            atom_slice_i=inp.series_n_i if len(inp.series_n_i) else None,
+            proj_coff_mat_i=inp.ele_n_i if len(inp.ele_n_i) else None,
+            series_n_corr=inp.series_n_corr if corr else None, ele_n_corr=inp.ele_n_corr if corr else None,
+            corr_info_matrix=inp.corr_matrix if corr else None, stream=st)
+        self.probes = torch.tensor(inp.probe.astype(np.complex64), device=dev)
+        self.prop = torch.tensor(optics.unit_complex64(inp.psf), device=dev)
+        self.aper = torch.tensor(inp.aper, device=dev)
+        self.aper2 = torch.tensor(inp.aper2, device=dev)
+        self.det = torch.zeros((self.L + 1, self.A, self.M * self.D), dtype=torch.float64, device=dev)
+        self.exit = (torch.zeros((self.M * self.D, self.N, self.N), dtype=torch.complex64, device=dev)
+                     if exit_wave else None)
+        self.job = engine.stem_job(self.N, self.NP, self.S, self.t.data_ptr(), self.probes.data_ptr(), self.M,
+                                   self.prop.data_ptr(), self.aper.data_ptr(), self.A, self.D, c.width_red, c.step,
+                                   c.begin_row - 1, c.begin_col - 1, self.det.data_ptr(),
+                                   aper2_ptr=self.aper2.data_ptr(), mid=sorted(c.mid_slices), wrap=wrap,
+                                   exit_ptr=None if self.exit is None else self.exit.data_ptr())
+        torch.cuda.current_stream().synchronize()   # inputs ready before another stream reads them
+
+    @property
+    def probe_slices(self):
+        return self.M * self.D * self.S
+
+    @property
+    def footprint(self):
+        return footprint_bytes(self.inp.cfg, self.M, self.A)
+
+    def images(self):
+        """(myresul [D, A], mid [D*A, L]) with the focal-spread weighting, as the mex returns them."""
+        det = self.det.cpu().numpy()
+        g = self.inp.gfsf
+        w = det.reshape(self.L + 1, self.A, self.M, self.D)
+        img = np.einsum("lamd,m->lad", w, g)
+        myresul = img[self.L].T.copy()
+        mid = np.zeros((self.D * self.A, self.L))
+        for a in range(self.A):
+            mid[a * self.D:(a + 1) * self.D, :] = img[:self.L, a, :].T
+        return myresul, mid
+
+
+def footprint_bytes(cfg, M, A):
+    """Device bytes a request keeps resident while in service (transmission stack, probes,
+    propagator, masks, outputs); wave slots come from the server's arena."""
+    n2, np2 = cfg.n * cfg.n, cfg.np_side * cfg.np_side
+    L = len(cfg.mid_slices)
+    return int(8 * cfg.n_slices * np2 + 8 * M * n2 + 8 * n2 + 16 * A * n2 + 8 * (L + 1) * A * M * cfg.depth)
+
+
+# ----------------------------------------------------------------- request mix
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float
+    inp: object
+    kind: str
+    admitted: float = None
+    done: float = None
+    job: DeviceJob = None
+    jid: int = None
+    result: tuple = None
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+GRID_MIX = ((256, 0.50), (512, 0.30), (1024, 0.15), (2048, 0.05))
+
+
+def mixed_request(rng, grids=GRID_MIX, slices=(10, 120), scan=(8, 64), cells=5):
+    """One seeded request of the mixed serving load: grid side from ``grids`` (side, probability),
+    thickness uniform in ``slices``, square scan of side uniform in ``scan``, Lobato or Peng 50/50,
+    SrTiO3 (``cells`` x ``cells`` cells laterally) or amorphous carbon of the same box, 200 kV,
+    20 mrad, ADF 60-200 mrad, periodic windows over one unit cell."""
+    sides, p = zip(*grids)
+    n = int(rng.choice(sides, p=np.asarray(p) / np.sum(p)))
+    S = int(rng.integers(slices[0], slices[1] + 1))
+    side = int(rng.integers(scan[0], scan[1] + 1))
+    flag = "l" if rng.random() < 0.5 else "p"
+    a = 3.905
+    dz = a / 2
+    box = cells * a
+    if rng.random() < 0.5:
+        kind = "perovskite"
+        atoms = specimen.perovskite(cells, cells, max(1, int(np.ceil(S / 2))), a=a)
+    else:
+        kind = "amorphous-C"
+        atoms = specimen.amorphous(box, box, 0.0, S * dz, Z=6, seed=int(rng.integers(1 << 30)))
+    dx = box / n
+    step = max(1, int(round(a / dx / side)))
+    cfg = stem.StemConfig(atoms=atoms, np_side=n, dx=dx, n=n, kv=200.0, alpha_mrad=20.0, slice_thickness=dz,
+                          n_slices=S, slice_origin=-dz / 4, paraflag=flag, detectors_mrad=((60.0, 200.0),),
+                          width_red=side, hight_red=side, step=step, begin_row=1, begin_col=1)
+    return cfg, kind
+
+
+# ----------------------------------------------------------------- serving loop
+
+def serve(requests, server, hbm_budget, build_stream=None, max_wall=600.0, on_done=None):
+    """Run the requests (sorted by ``arrival``, seconds after the start) through ``server``.
+
+    A request is admitted when it has arrived and its footprint fits ``hbm_budget`` next to
+    the requests in service (first come, first served).  Returns (wall seconds, peak resident
+    bytes); each request gets ``admitted``, ``done`` (seconds) and ``result`` (mex-shaped images)."""
+    import torch
+    waiting = deque(sorted(requests, key=lambda r: r.arrival))
+    arrived = deque()
+    in_service = {}
+    resident = peak = 0
+    t0 = time.perf_counter()
+    while waiting or arrived or in_service:
+        now = time.perf_counter() - t0
+        if now > max_wall:
+            raise RuntimeError("serving loop exceeded %.0f s" % max_wall)
+        while waiting and waiting[0].arrival <= now:
+            arrived.append(waiting.popleft())
+        while arrived:
+            r = arrived[0]
+            fp = footprint_bytes(r.inp.cfg, len(r.inp.gfsf), r.inp.aper.shape[0])
+            if in_service and resident + fp > hbm_budget:
+                break
+            arrived.popleft()
+            r.admitted = time.perf_counter() - t0
+            r.job = DeviceJob(r.inp, stream=build_stream)
+            r.jid = server.submit(r.job.job)
+            in_service[r.jid] = r
+            resident += fp
+            peak = max(peak, resident)
+        if in_service:
+            server.step()
+            for jid in server.poll():
+                r = in_service.pop(jid)
+                r.result = r.job.images()
+                r.done = time.perf_counter() - t0
+                resident -= footprint_bytes(r.inp.cfg, len(r.inp.gfsf), r.inp.aper.shape[0])
+                if on_done:
+                    on_done(r)
+                r.job = None
+        elif waiting:
+            time.sleep(max(0.0, min(0.01, waiting[0].arrival - (time.perf_counter() - t0))))
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, peak
