@@ -65,6 +65,21 @@ def cpu_baseline(t_dev, inp, positions, cores):
     return len(positions) * cfg.n_slices / wall, wall, np.asarray(res)
 
 
+def measured_traffic(kname, n, workload):
+    """HBM bytes per dispatch of the kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (copy-calibrated read + write GiB columns), or None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", "stem_passes_%d_%s_traffic.txt" % (n, workload))
+    if not os.path.exists(path):
+        return None
+    row = kname.replace(">", ", false>") if kname.startswith("k_pass_cols") else kname
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(row + " "):
+                f = line[len(row):].split()
+                return (float(f[1]) + float(f[3])) * 2**30
+    return None
+
+
 def run_serving(n_requests, rate, seed, arena_gib, budget_gib):
     """Poisson arrivals of mixed-size STEM requests (grids 256^2 50%, 512^2 30%, 1024^2 15%,
     2048^2 5%; 10-120 slices; scans 8x8-64x64; Lobato or Peng; SrTiO3 or amorphous C) through the
@@ -92,6 +107,41 @@ def run_serving(n_requests, rate, seed, arena_gib, budget_gib):
             "arena_GiB": arena_gib, "hbm_budget_GiB": budget_gib, "seed": seed}
 
 
+def run_uneven(n_jobs, seed, arena_gib):
+    """Uneven batch: n_jobs seeded perovskite jobs of mixed grid sides (256^2-1024^2, 768^2 with a
+    radix-3 stage), thickness, defocus and scan size, all in one served run, against the same jobs
+    run one at a time through the one-shot entry point (each job fully batched)."""
+    import torch
+    from totem_amd import engine, serve, stem
+    rng = np.random.default_rng(seed)
+    cfgs = [serve.uneven_request(rng) for _ in range(n_jobs)]
+    jobs = [serve.DeviceJob(stem.prepare(c)) for c in cfgs]
+    work = sum(dj.probe_slices for dj in jobs)
+    srv = engine.Server(int(arena_gib * 2**30), 8, 16)
+    try:
+        serve.run_all(srv, jobs[:2])          # warm the per-size plans and streams
+        for dj in jobs:
+            dj.det.zero_()
+        wall = serve.run_all(srv, jobs)
+    finally:
+        srv.close()
+    served = [dj.det.cpu().numpy().copy() for dj in jobs]
+    st = torch.cuda.current_stream().cuda_stream
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for dj in jobs:
+        engine.stem_run(dj.job, dj.M * dj.D, stream=st)
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t0
+    same = all(np.array_equal(a, dj.det.cpu().numpy()) for a, dj in zip(served, jobs))
+    sides = {}
+    for c in cfgs:
+        sides["%d" % c.n] = sides.get("%d" % c.n, 0) + 1
+    return {"jobs": n_jobs, "grids": sides, "probe_slices": int(work), "served_wall_s": wall,
+            "served_probe_slices_per_s": work / wall, "one_job_at_a_time_wall_s": seq,
+            "one_job_at_a_time_probe_slices_per_s": work / seq, "bit_identical": same, "seed": seed}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +153,7 @@ def main():
     ap.add_argument("--cpu-probes", type=int, default=16)
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--uneven-jobs", type=int, default=48, help="jobs of the uneven-batch leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the mixed serving load (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate, requests/s")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -223,11 +274,7 @@ def main():
     else:
         kname, kavg, kbytes = "k_pass_cols<%d>" % N, cols_avg, bytes_cols
     achieved = kbytes / (kavg * 1e-3) / 1e9 if kavg > 0 else None
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get(kname)
+    traffic = measured_traffic(kname, N, args.workload)
 
     out = {
         "metric": "probe.slice/s at 1024^2 FP32 (batched vs plain one-probe-per-launch form)",
@@ -264,9 +311,12 @@ def main():
                                          "oracle (propagation only, transmission taken from the GPU build), "
                                          "%.1f s wall" % (len(positions), S, n_cfg - 1, cwall)}
         out["oracle_max_rel_err"] = rel
-    if args.serve_requests > 0:
+    if args.uneven_jobs > 0 or args.serve_requests > 0:
         del ws, t, probes, det
         torch.cuda.empty_cache()
+    if args.uneven_jobs > 0:
+        out["uneven_batch"] = run_uneven(args.uneven_jobs, seed=1004, arena_gib=64)
+    if args.serve_requests > 0:
         out["serving"] = run_serving(args.serve_requests, args.serve_rate, seed=1005, arena_gib=64, budget_gib=160)
     print(json.dumps(out))
 

@@ -136,6 +136,42 @@ def mixed_request(rng, grids=GRID_MIX, slices=(10, 120), scan=(8, 64), cells=5):
     return cfg, kind
 
 
+def uneven_request(rng):
+    """One job of the uneven batch: SrTiO3 with a seeded supercell of 3-8 cells laterally, grid side
+    from {256, 512, 768, 1024}, 10-60 slices, defocus uniform in [-10, 10] nm, square scan of side
+    8-32 over one unit cell, 200 kV, 20 mrad, ADF 60-200 mrad, Lobato."""
+    n = int(rng.choice((256, 512, 768, 1024)))
+    S = int(rng.integers(10, 61))
+    side = int(rng.integers(8, 33))
+    cells = int(rng.integers(3, 9))
+    a = 3.905
+    dz = a / 2
+    atoms = specimen.perovskite(cells, cells, max(1, int(np.ceil(S / 2))), a=a)
+    dx = cells * a / n
+    step = max(1, int(round(a / dx / side)))
+    ab = optics.Aberrations(C1=float(rng.uniform(-10.0, 10.0)))
+    return stem.StemConfig(atoms=atoms, np_side=n, dx=dx, n=n, kv=200.0, alpha_mrad=20.0, slice_thickness=dz,
+                           n_slices=S, slice_origin=-dz / 4, paraflag="l", aberrations=ab,
+                           detectors_mrad=((60.0, 200.0),), width_red=side, hight_red=side, step=step,
+                           begin_row=1, begin_col=1)
+
+
+def run_all(server, jobs, max_wall=600.0):
+    """Submit every DeviceJob at once and step the server until all are done; returns wall seconds."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ids = {server.submit(dj.job) for dj in jobs}
+    done = set()
+    while not ids <= done:
+        server.step()
+        done.update(server.poll())
+        if time.perf_counter() - t0 > max_wall:
+            raise RuntimeError("server run exceeded %.0f s" % max_wall)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
 # ----------------------------------------------------------------- serving loop
 
 def serve(requests, server, hbm_budget, build_stream=None, max_wall=600.0, on_done=None):
