@@ -60,6 +60,48 @@ template <int N> struct ColCfg {
   static constexpr int THREADS = T * C;
 };
 
+#ifndef TOTEM_TILED
+#define TOTEM_TILED 1  // in-flight waves in the tiled layout below (0: row-major)
+#endif
+
+// Layout of an in-flight wave [N][N] (the probe, the transmission, the propagator
+// and the outputs stay row-major).  Tiled: TR x TC tiles of 16 elements (one
+// 128-byte line), tiles row-major, elements row-major inside a tile.  A rows-pass
+// workgroup (TR rows at a time) then streams whole lines, and a columns-pass
+// wave (C >= TC columns x 64/C consecutive rows) touches whole lines instead of
+// C x 8-byte segments of 64/C rows.  Element (r, c) sits at
+//   ((r / TR) * (N / TC) + c / TC) * 16 + (r % TR) * TC + c % TC.
+template <int N> struct WaveLay {
+  static constexpr int TR = TOTEM_TILED ? (RowCfg<N>::RW < 4 ? RowCfg<N>::RW : 4) : 1;
+  static constexpr int TC = TOTEM_TILED ? 16 / TR : 1;
+  static_assert(!TOTEM_TILED || (TR * TC == 16 && N % TC == 0 && Plan<N>::T % TC == 0 &&
+                                 Plan<N>::T % TR == 0 && RowCfg<N>::RW % TR == 0), "tile shape");
+  __device__ static int at(int r, int c) {
+    if constexpr (!TOTEM_TILED) return r * N + c;
+    return ((r / TR) * (N / TC) + c / TC) * 16 + (r % TR) * TC + c % TC;
+  }
+  // element stride between (r, c + T*v) and (r, c + T*(v+1)) for T a multiple of TC
+  static constexpr int row_step(int T) { return TOTEM_TILED ? 16 * (T / TC) : T; }
+};
+
+#ifndef TOTEM_ROWS_LANE_TILED
+#define TOTEM_ROWS_LANE_TILED 0
+#endif
+// Rows-pass lane -> (row of the workgroup, lane within the row).  Default: T
+// consecutive lanes per row, so the LDS exchanges of a wave stay within one row
+// (conflict-free); the TR rows of a tile are read by TR waves of the workgroup at
+// the same time and share each line in the caches.  With TOTEM_ROWS_LANE_TILED the
+// 64 lanes of a wave cover TR rows x 64/TR columns (whole tiles per instruction);
+// the LDS rows are then padded by ROWS_LDS_PAD float2 so that the TR rows of a wave
+// fall on different banks (measured: conflict-free, but no faster at 1024^2).
+constexpr int ROWS_LDS_PAD = TOTEM_ROWS_LANE_TILED ? 4 : 0;
+template <int N> __device__ __forceinline__ void row_lane(int tid, int& lr, int& t) {
+  constexpr int T = RowCfg<N>::T, TR = TOTEM_ROWS_LANE_TILED ? WaveLay<N>::TR : 1;
+  const int q = tid / (TR * T), rem = tid - q * (TR * T);
+  lr = q * TR + rem % TR;
+  t = rem / TR;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
 // With the remap (unit_map below), XCD x works through the groups
 // [x*G/8, (x+1)*G/8) of every probe in order: neighbouring column groups (which
@@ -109,8 +151,9 @@ __device__ __forceinline__ void unit_map(int bid, int u, int G, int& item, int& 
 struct RowUnit {
   bool active;
   int slice;
-  const float2* src;   // wave read: probe (slice 0) or psi
-  float2* dst;         // psi row
+  const float2* src;   // wave read: probe (slice 0, row-major) or the item's wave
+  int sstep;           // element stride of src between consecutive v
+  float2* dst;         // the item's wave (WaveLay)
   const float2* trow;  // transmission row
   int c0, np;
 };
@@ -128,8 +171,9 @@ __device__ __forceinline__ RowUnit row_unit(const ItemDev* __restrict__ items, c
   r.slice = step - it.start_step;
   r.active = r.slice >= 0 && r.slice < jb.S;
   const int row = rg * RW + lr;
-  r.dst = it.wave + (size_t)row * N + t;
+  r.dst = it.wave + WaveLay<N>::at(row, t);
   r.src = r.slice == 0 ? it.probe + (size_t)row * N + t : r.dst;
+  r.sstep = r.slice == 0 ? Cfg::T : WaveLay<N>::row_step(Cfg::T);
   r.np = jb.np;
   int pr = it.r0 + row;
   if (pr >= r.np) pr -= r.np;  // only reachable with wrap (validated on the host)
@@ -143,16 +187,18 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS, TOTEM_ROWS_MINWAVES)
 k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step, const float2* __restrict__ tw) {
   using Cfg = RowCfg<N>;
   constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, K = TOTEM_ROWS_K;
-  __shared__ float2 lds[Cfg::NBUF * RW * Cfg::LL];
-  float2* const lds1 = lds + (Cfg::NBUF - 1) * RW * Cfg::LL;
-  const int lr = threadIdx.x / T;
-  const int t = threadIdx.x - lr * T;
-  const RowLay lay{lr * Cfg::LL};
+  constexpr int LLP = Cfg::LL + ROWS_LDS_PAD;
+  __shared__ float2 lds[Cfg::NBUF * RW * LLP];
+  float2* const lds1 = lds + (Cfg::NBUF - 1) * RW * LLP;
+  int lr, t;
+  row_lane<N>(threadIdx.x, lr, t);
+  const RowLay lay{lr * LLP};
+  constexpr int DSTEP = WaveLay<N>::row_step(T);
   float2 xa[E], xb[E];
   RowUnit ua = row_unit<N, K>(items, jobs, step, 0, lr, t);
   if (ua.active) {
 #pragma unroll
-    for (int v = 0; v < E; ++v) xa[v] = gld(ua.src + T * v);
+    for (int v = 0; v < E; ++v) xa[v] = gld(ua.src + ua.sstep * v);
   }
 #pragma unroll
   for (int u = 0; u < K; ++u) {
@@ -161,7 +207,7 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
       ub = row_unit<N, K>(items, jobs, step, u + 1, lr, t);
       if (ub.active) {
 #pragma unroll
-        for (int v = 0; v < E; ++v) xb[v] = gld(ub.src + T * v);
+        for (int v = 0; v < E; ++v) xb[v] = gld(ub.src + ub.sstep * v);
       }
     }
     if (ua.active) {  // uniform over the workgroup
@@ -192,7 +238,7 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
       }
       block_fft<N, -1, Cfg::PP, next_start<N, Cfg::PP>(0)>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
-      for (int v = 0; v < E; ++v) gst(ua.dst + T * v, xa[v]);
+      for (int v = 0; v < E; ++v) gst(ua.dst + DSTEP * v, xa[v]);
       if (u + 1 < K) __syncthreads();  // LDS reuse by the next unit
     }
     if (u + 1 < K) {
@@ -288,7 +334,10 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   constexpr int kStep = T * N * (int)sizeof(float2);
   float2 xa[E], xb[E];
   ColUnit ua = col_unit<N, K>(items, jobs, step, 0, DET);
-  int voffa = (t * N + ua.cg * C + c) * (int)sizeof(float2);
+  // byte offsets of (row t, column): in the wave (WaveLay) and in row-major arrays (H, exit wave);
+  // both advance by T*N elements per v
+  int voffa = WaveLay<N>::at(t, ua.cg * C + c) * (int)sizeof(float2);
+  int hoffa = (t * N + ua.cg * C + c) * (int)sizeof(float2);
   if (ua.active) {
 #pragma unroll
     for (int v = 0; v < E; ++v) xa[v] = bld(ua.wr, voffa, v * kStep);
@@ -296,10 +345,11 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
 #pragma unroll
   for (int u = 0; u < K; ++u) {
     ColUnit ub;
-    int voffb = 0;
+    int voffb = 0, hoffb = 0;
     if (u + 1 < K) {
       ub = col_unit<N, K>(items, jobs, step, u + 1, DET);
-      voffb = (t * N + ub.cg * C + c) * (int)sizeof(float2);
+      voffb = WaveLay<N>::at(t, ub.cg * C + c) * (int)sizeof(float2);
+      hoffb = (t * N + ub.cg * C + c) * (int)sizeof(float2);
       if (ub.active) {
 #pragma unroll
         for (int v = 0; v < E; ++v) xb[v] = bld(ub.wr, voffb, v * kStep);
@@ -312,11 +362,11 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
       float2 hv[TOTEM_COLS_HPREFETCH ? E : 1];
       if constexpr (TOTEM_COLS_HPREFETCH) {
 #pragma unroll
-        for (int v = 0; v < E; ++v) hv[v] = bld(hr, voffa, v * kStep);
+        for (int v = 0; v < E; ++v) hv[v] = bld(hr, hoffa, v * kStep);
       }
       block_fft<N, -1, Cfg::PP, 0>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
-      for (int v = 0; v < E; ++v) xa[v] = cmul(xa[v], TOTEM_COLS_HPREFETCH ? hv[v] : bld(hr, voffa, v * kStep));
+      for (int v = 0; v < E; ++v) xa[v] = cmul(xa[v], TOTEM_COLS_HPREFETCH ? hv[v] : bld(hr, hoffa, v * kStep));
       bool keep = true;
       if constexpr (DET) {
         if (ua.mid >= 0) {
@@ -332,7 +382,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
           if (jb.exit) {
             const auto er = buf_rsrc(jb.exit + (size_t)ua.out * N * N, kBytes);
 #pragma unroll
-            for (int v = 0; v < E; ++v) bst(er, voffa, v * kStep, xa[v]);
+            for (int v = 0; v < E; ++v) bst(er, hoffa, v * kStep, xa[v]);
           }
         }
       }
@@ -346,6 +396,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
     if (u + 1 < K) {
       ua = ub;
       voffa = voffb;
+      hoffa = hoffb;
 #pragma unroll
       for (int v = 0; v < E; ++v) xa[v] = xb[v];
     }
