@@ -149,6 +149,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
     ap.add_argument("--graph", type=int, default=0, help="capture the slice loop in a hipGraph")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="probes in flight per launch (0: the engine's cache-fit batch)")
     ap.add_argument("--plain-probes", type=int, default=16)
     ap.add_argument("--cpu-probes", type=int, default=16)
     ap.add_argument("--cpu-cores", type=int, default=16)
@@ -196,7 +198,7 @@ def main():
     job = engine.stem_job(N, NP, S, t.data_ptr(), probes.data_ptr(), M, prop.data_ptr(), aper.data_ptr(), A, D,
                           cfg.width_red, cfg.step, cfg.begin_row - 1, cfg.begin_col - 1, det.data_ptr(),
                           aper2_ptr=aper2.data_ptr(), wrap=True)
-    batch = M * D
+    batch = min(M * D, args.batch if args.batch > 0 else engine.preferred_batch(N))
     ws_bytes = engine.stem_workspace_bytes(job, batch)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
@@ -228,13 +230,17 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        step(rows[args.warmup + k], timing=not args.graph)
+        step(rows[args.warmup + k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    rows_ms, rows_n, cols_ms, cols_n = engine.kernel_times(reset=True)
-    det_last = det.cpu().numpy()[0]
     probe_slices = args.steps * M * D * S
     value = probe_slices / wall
+
+    # per-kernel durations: the last configuration once more with HIP events around every pass
+    # (on the launch stream; the events synchronise per batch, so this run is not the timed one)
+    engine.stem_run(job, batch, workspace_ptr=ws.data_ptr(), workspace_bytes=ws_bytes, stream=stream, timing=True)
+    torch.cuda.synchronize()
+    rows_ms, rows_n, cols_ms, cols_n = engine.kernel_times(reset=True)
 
     # potential build alone (per phonon configuration)
     torch.cuda.synchronize()
@@ -289,7 +295,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded specimen and phonon snapshots)",
-        "config": {"workload": name, "grid": N, "slices": S, "probes_per_step": M * D,
+        "config": {"workload": name, "grid": N, "slices": S, "probes_per_step": M * D, "probes_per_launch": batch,
                    "phonon_configs_timed": args.steps, "paraflag": cfg.paraflag, "graph": bool(args.graph)},
         "plain": {"value": plain_value, "probes": pp,
                   "speedup_batched_vs_plain": value / plain_value if plain_value else None,
@@ -315,9 +321,9 @@ def main():
         del ws, t, probes, det
         torch.cuda.empty_cache()
     if args.uneven_jobs > 0:
-        out["uneven_batch"] = run_uneven(args.uneven_jobs, seed=1004, arena_gib=64)
+        out["uneven_batch"] = run_uneven(args.uneven_jobs, seed=1004, arena_gib=0.25)
     if args.serve_requests > 0:
-        out["serving"] = run_serving(args.serve_requests, args.serve_rate, seed=1005, arena_gib=64, budget_gib=160)
+        out["serving"] = run_serving(args.serve_requests, args.serve_rate, seed=1005, arena_gib=0.25, budget_gib=160)
     print(json.dumps(out))
 
 

@@ -139,6 +139,10 @@ typedef struct totem_stem_job {
 #define TOTEM_RUN_GRAPH 1     /* capture the slice loop of a batch in a hipGraph and replay it */
 #define TOTEM_RUN_TIMING 2    /* time every rows/columns pass with HIP events (no graph; synchronises) */
 
+/* Probes in flight per launch that keep their n x n waves inside the GPU's
+ * memory-side cache (the batch myProjectfun uses; no counterpart in the
+ * reference, which fills free memory, cuda_source_lobato.cu:1249-1260). */
+int totem_stem_preferred_batch(int n);
 /* Device bytes of workspace totem_stem_run needs for `batch` probes in flight. */
 size_t totem_stem_workspace_bytes(const totem_stem_job *job, int batch);
 /* Runs all M*D probes of the job, `batch` at a time (batch = 1 is the plain

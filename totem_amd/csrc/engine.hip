@@ -383,19 +383,22 @@ struct StemWorkspace {
   int* layer_hit;
 };
 
-static size_t stem_ws_bytes(int n, int S, int A, int L, int batch) {
+// Workspace of `batch` probes in flight out of `total`: waves and detector
+// partials per slot, the work items of every probe of the job, and `batch` more
+// items at a fixed address for graph replays.
+static size_t stem_ws_bytes(int n, int S, int A, int L, int batch, long total) {
   const int G = detector_groups(n);
   size_t b = 0;
   b += align_up((size_t)batch * n * n * sizeof(float2));
   b += align_up((size_t)batch * (L + 1) * A * G * sizeof(double));
-  b += align_up((size_t)batch * sizeof(ItemDev));
+  b += align_up((size_t)(total + batch) * sizeof(ItemDev));
   b += align_up(sizeof(JobDev));
   b += align_up((size_t)std::max(S, 1) * sizeof(int));
   b += align_up((size_t)std::max(L, 1) * sizeof(int));
   return b;
 }
 
-static StemWorkspace carve(void* base, int n, int S, int A, int L, int batch) {
+static StemWorkspace carve(void* base, int n, int S, int A, int L, int batch, long total) {
   const int G = detector_groups(n);
   char* p = static_cast<char*>(base);
   StemWorkspace w;
@@ -404,7 +407,7 @@ static StemWorkspace carve(void* base, int n, int S, int A, int L, int batch) {
   w.partial = reinterpret_cast<double*>(p);
   p += align_up((size_t)batch * (L + 1) * A * G * sizeof(double));
   w.items = reinterpret_cast<ItemDev*>(p);
-  p += align_up((size_t)batch * sizeof(ItemDev));
+  p += align_up((size_t)(total + batch) * sizeof(ItemDev));
   w.job = reinterpret_cast<JobDev*>(p);
   p += align_up(sizeof(JobDev));
   w.mid_of_slice = reinterpret_cast<int*>(p);
@@ -460,6 +463,14 @@ void window_origin(const totem_stem_job& j, int d, int& r0, int& c0) {
   c0 = (int)c;
 }
 
+// Probes in flight that keep a batch's waves (n^2 complex64 each) within the
+// 256 MB memory-side cache of the MI355X, leaving room for the slice and the
+// propagator: then every slice step's wave traffic stays in that cache and only
+// the transmission slices stream from HBM.  Measured optimum 192-224 MiB of waves
+// for 256^2 to 2048^2 grids (1.1-1.2x the throughput of HBM-resident batches).
+constexpr size_t kCacheWaveBytes = 208ull << 20;
+int preferred_batch(int n) { return (int)std::max<size_t>(1, kCacheWaveBytes / ((size_t)n * n * sizeof(float2))); }
+
 // Per-kernel durations of the last timed run (TOTEM_RUN_TIMING), from HIP events
 // recorded on the launch stream around every rows/columns pass.
 struct KernelTimes {
@@ -474,7 +485,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   const long total = (long)M * D;
   require(batch > 0, "batch must be positive");
   batch = (int)std::min<long>(batch, total);
-  const size_t need = stem_ws_bytes(n, S, A, L, batch);
+  const size_t need = stem_ws_bytes(n, S, A, L, batch, total);
   DevBuf own;
   if (!ws) {
     own.alloc(need);
@@ -482,7 +493,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   } else {
     require(ws_bytes >= need, "workspace too small: need " + std::to_string(need) + " bytes", TOTEM_ERR_MEMORY);
   }
-  StemWorkspace w = carve(ws, n, S, A, L, batch);
+  StemWorkspace w = carve(ws, n, S, A, L, batch, total);
   const int G = detector_groups(n);
   tw_table(n);
   std::vector<int> mos, hit;
@@ -506,44 +517,51 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   HIP_OK(hipMemcpyAsync(w.mid_of_slice, mos.data(), S * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(w.layer_hit, hit.data(), hit.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_OK(hipStreamSynchronize(st));  // host staging buffers above are locals
+  // every probe's work item, uploaded once: chunk c runs items [c*batch, ...) in
+  // wave slots 0..batch-1, which consecutive chunks reuse in stream order
   const float2* probes = static_cast<const float2*>(j.probes);
-  std::vector<ItemDev> items(batch);
+  std::vector<ItemDev> items(total);
+  for (long o = 0; o < total; ++o) {
+    const int m = (int)(o / D), d = (int)(o % D);
+    const long k = o % batch;
+    ItemDev& it = items[o];
+    it.probe = probes + (size_t)m * n * n;
+    it.job = 0;
+    window_origin(j, d, it.r0, it.c0);
+    it.start_step = 0;
+    it.wave = w.psi + (size_t)k * n * n;
+    it.part = w.partial + (size_t)k * (L + 1) * A * G;
+    it.out = (int)o;
+  }
+  ItemDev* const stage = w.items + total;  // graph replays read the chunk's items here
+  HIP_OK(hipMemcpyAsync(w.items, items.data(), total * sizeof(ItemDev), hipMemcpyHostToDevice, st));
+  HIP_OK(hipStreamSynchronize(st));  // the host list is a local
+  auto record = [&](const ItemDev* it, int cnt) {
+    for (int s = 0; s < S; ++s) {
+      pass_rows(n, st, it, cnt, w.job, s);
+      pass_cols(n, st, it, cnt, w.job, s, s == S - 1 || mos[s] >= 0);
+    }
+    reduce_det(st, it, nullptr, cnt, w.job, G, L + 1, A);
+  };
+  std::vector<hipEvent_t> ev;
+  if (flags & TOTEM_RUN_TIMING) {
+    ev.resize(2 * S + 1);
+    for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+  }
   hipGraphExec_t gexec = nullptr;
   int graph_count = -1;
   for (long i0 = 0; i0 < total; i0 += batch) {
     const int cnt = (int)std::min<long>(batch, total - i0);
-    if (i0 > 0) HIP_OK(hipStreamSynchronize(st));  // the previous chunk still reads w.items
-    for (int k = 0; k < cnt; ++k) {
-      const long o = i0 + k;
-      const int m = (int)(o / D), d = (int)(o % D);
-      ItemDev& it = items[k];
-      it.probe = probes + (size_t)m * n * n;
-      it.job = 0;
-      window_origin(j, d, it.r0, it.c0);
-      it.start_step = 0;
-      it.wave = w.psi + (size_t)k * n * n;
-      it.part = w.partial + (size_t)k * (L + 1) * A * G;
-      it.out = (int)o;
-    }
-    HIP_OK(hipMemcpyAsync(w.items, items.data(), cnt * sizeof(ItemDev), hipMemcpyHostToDevice, st));
-    auto record = [&]() {
-      for (int s = 0; s < S; ++s) {
-        pass_rows(n, st, w.items, cnt, w.job, s);
-        pass_cols(n, st, w.items, cnt, w.job, s, s == S - 1 || mos[s] >= 0);
-      }
-      reduce_det(st, w.items, nullptr, cnt, w.job, G, L + 1, A);
-    };
+    const ItemDev* ci = w.items + i0;
     if (flags & TOTEM_RUN_TIMING) {
-      std::vector<hipEvent_t> ev(2 * S + 1);
-      for (auto& e : ev) HIP_OK(hipEventCreate(&e));
       HIP_OK(hipEventRecord(ev[0], st));
       for (int s = 0; s < S; ++s) {
-        pass_rows(n, st, w.items, cnt, w.job, s);
+        pass_rows(n, st, ci, cnt, w.job, s);
         HIP_OK(hipEventRecord(ev[2 * s + 1], st));
-        pass_cols(n, st, w.items, cnt, w.job, s, s == S - 1 || mos[s] >= 0);
+        pass_cols(n, st, ci, cnt, w.job, s, s == S - 1 || mos[s] >= 0);
         HIP_OK(hipEventRecord(ev[2 * s + 2], st));
       }
-      reduce_det(st, w.items, nullptr, cnt, w.job, G, L + 1, A);
+      reduce_det(st, ci, nullptr, cnt, w.job, G, L + 1, A);
       HIP_OK(hipEventSynchronize(ev[2 * S]));
       for (int s = 0; s < S; ++s) {
         float a = 0.f, b = 0.f;
@@ -554,13 +572,13 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
         g_times.rows_n += 1;
         g_times.cols_n += 1;
       }
-      for (auto& e : ev) HIP_OK(hipEventDestroy(e));
     } else if (flags & TOTEM_RUN_GRAPH) {
+      HIP_OK(hipMemcpyAsync(stage, ci, cnt * sizeof(ItemDev), hipMemcpyDeviceToDevice, st));
       if (graph_count != cnt) {
         if (gexec) HIP_OK(hipGraphExecDestroy(gexec));
         hipGraph_t g;
         HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        record();
+        record(stage, cnt);
         HIP_OK(hipStreamEndCapture(st, &g));
         HIP_OK(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0));
         HIP_OK(hipGraphDestroy(g));
@@ -568,10 +586,11 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
       }
       HIP_OK(hipGraphLaunch(gexec, st));
     } else {
-      record();
+      record(ci, cnt);
     }
     HIP_OK(hipGetLastError());
   }
+  for (auto& e : ev) HIP_OK(hipEventDestroy(e));
   if (gexec) {
     HIP_OK(hipStreamSynchronize(st));
     HIP_OK(hipGraphExecDestroy(gexec));
@@ -716,14 +735,15 @@ static void project_impl(const ProjectArgs& a, int paraflag, bool potential_only
   j.aper = daper.as<double>();
   j.aper2 = daper2.as<double>();
   j.det = ddet.as<double>();
-  // probes in flight: bounded by free HBM (the reference sizes its batches the same way,
-  // cuda_source_lobato.cu:1249-1260)
+  // probes in flight: as many as keep their waves inside the memory-side cache
+  // (preferred_batch), and never more than half the free HBM (the reference sizes
+  // its batches by free memory alone, cuda_source_lobato.cu:1249-1260)
   size_t fr = 0, tot = 0;
   HIP_OK(hipMemGetInfo(&fr, &tot));
-  const size_t per = stem_ws_bytes(n, S, A, L, 2) - stem_ws_bytes(n, S, A, L, 1);
+  const size_t per = stem_ws_bytes(n, S, A, L, 2, 2) - stem_ws_bytes(n, S, A, L, 1, 1);
   long batch = (long)(fr / 2 / std::max<size_t>(per, 1));
-  batch = std::max<long>(1, std::min<long>(batch, std::min<long>((long)M * D, 16384)));
-  DevBuf ws(stem_ws_bytes(n, S, A, L, (int)batch));
+  batch = std::max<long>(1, std::min<long>(batch, std::min<long>((long)M * D, preferred_batch(n))));
+  DevBuf ws(stem_ws_bytes(n, S, A, L, (int)batch, (long)M * D));
   run_stem_impl(j, (int)batch, 0, ws.p, ws.bytes, st);
   std::vector<double> det(det_n);
   HIP_OK(hipMemcpyAsync(det.data(), ddet.p, det_n * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -788,9 +808,12 @@ int totem_build_transmission(const totem_potential_desc* desc, void* t_dev, doub
   });
 }
 
+int totem_stem_preferred_batch(int n) { return n > 0 ? totem::preferred_batch(n) : 0; }
+
 size_t totem_stem_workspace_bytes(const totem_stem_job* job, int batch) {
   if (!job || batch <= 0 || !size_supported(job->n)) return 0;
-  return stem_ws_bytes(job->n, job->slices, job->n_aper, std::max(job->n_mid, 0), batch);
+  return stem_ws_bytes(job->n, job->slices, job->n_aper, std::max(job->n_mid, 0), batch,
+                       (long)job->n_probes * job->depth);
 }
 
 int totem_stem_run(const totem_stem_job* job, int batch, int flags, void* workspace, size_t ws_bytes, void* stream) {

@@ -77,6 +77,8 @@ void validate_job(const totem_stem_job& j);
 // Mid-layer walk of the reference (cuda_source_lobato.cu:1340-1385): mos[s] is the
 // mid layer written after slice s (-1: none), hit[l] = 1 if layer l is written.
 void mid_walk(const totem_stem_job& j, std::vector<int>& mos, std::vector<int>& hit);
+// Probes in flight whose waves fit the memory-side cache (the default batch).
+int preferred_batch(int n);
 // Window origin (row, column) of scan position d (periodic when j.wrap).
 void window_origin(const totem_stem_job& j, int d, int& r0, int& c0);
 

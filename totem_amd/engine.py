@@ -35,7 +35,7 @@ EXPORTS = (
     "totem_project_lobato", "totem_project_peng", "totem_potential_lobato", "totem_potential_peng",
     "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
     "totem_stem_workspace_bytes", "totem_stem_run", "totem_kernel_times", "totem_fft2d",
-    "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
+    "totem_stem_preferred_batch", "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
     "totem_server_destroy",
 )
 
@@ -116,6 +116,8 @@ def lib():
         L.totem_kernel_times.restype = None
         L.totem_fft2d.argtypes = [_VP, _I, _I, _I, _VP]
         L.totem_fft2d.restype = _I
+        L.totem_stem_preferred_batch.argtypes = [_I]
+        L.totem_stem_preferred_batch.restype = _I
         L.totem_server_create.argtypes = [ctypes.c_size_t, _I, _I, ctypes.POINTER(_VP)]
         L.totem_server_submit.argtypes = [_VP, ctypes.POINTER(StemJob), ctypes.POINTER(ctypes.c_longlong)]
         L.totem_server_step.argtypes = [_VP, ctypes.POINTER(_I)]
@@ -274,6 +276,11 @@ def stem_job(n, np_side, slices, t_ptr, probes_ptr, n_probes, prop_ptr, aper_ptr
     j.det = det_ptr
     j.exit_wave = exit_ptr
     return j
+
+
+def preferred_batch(n):
+    """Probes in flight whose n x n waves fit the GPU's memory-side cache."""
+    return int(lib().totem_stem_preferred_batch(int(n)))
 
 
 def stem_workspace_bytes(job, batch):
