@@ -67,7 +67,7 @@ def cpu_baseline(t_dev, inp, positions, cores):
 
 def measured_traffic(kname, n, workload):
     """HBM bytes per dispatch of the kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (copy-calibrated read + write GiB columns), or None when no summary covers this workload."""
+    (copy-calibrated read + write MiB columns), or None when no summary covers this workload."""
     path = os.path.join(ROOT, "profiles", "stem_passes_%d_%s_traffic.txt" % (n, workload))
     if not os.path.exists(path):
         return None
@@ -76,7 +76,7 @@ def measured_traffic(kname, n, workload):
         for line in fh:
             if line.startswith(row + " "):
                 f = line[len(row):].split()
-                return (float(f[1]) + float(f[3])) * 2**30
+                return (float(f[1]) + float(f[3])) * 2**20
     return None
 
 
@@ -149,6 +149,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
     ap.add_argument("--graph", type=int, default=0, help="capture the slice loop in a hipGraph")
+    ap.add_argument("--scan", type=int, default=64, help="scan side of the 1024^2 workload (64: 64x64 probes)")
     ap.add_argument("--batch", type=int, default=0,
                     help="probes in flight per launch (0: the engine's cache-fit batch)")
     ap.add_argument("--plain-probes", type=int, default=16)
@@ -174,10 +175,10 @@ def main():
     engine.lib()
     dev = torch.device("cuda:0")
     if args.workload == "c2":
-        cfg = stem.perovskite_c2()
+        cfg = stem.perovskite_c2(scan=args.scan)
         sig = stem.C2_PHONON_SIGMA
-        name = ("SrTiO3 20x20x40 cells frozen phonon, Peng+corr, 1024^2, 80 slices, 64x64 probes, "
-                "300 kV 24 mrad Cs 1um C1 -5nm, ADF 70-200 + BF 0-10 mrad")
+        name = ("SrTiO3 20x20x40 cells frozen phonon, Peng+corr, 1024^2, 80 slices, %dx%d probes, "
+                "300 kV 24 mrad Cs 1um C1 -5nm, ADF 70-200 + BF 0-10 mrad" % (args.scan, args.scan))
     else:
         cfg = stem.perovskite_c1()
         sig = {}
