@@ -194,3 +194,28 @@ def test_mex_gateway_argument_mapping(tmp_path):
     assert "nrhs40=Invalid number of input arguments" in out
     out = subprocess.run([str(exe), "fail"], capture_output=True, text=True, check=True).stdout
     assert "error=engine failure" in out
+
+
+def test_diamond_110_repeat_and_bond_length():
+    from scipy.spatial import cKDTree
+    from totem_amd import specimen
+    a = 5.431
+    x = specimen.diamond_110(10 * a / np.sqrt(2), 7 * a, 10 * a / np.sqrt(2), a=a)
+    assert len(x) == 4 * 10 * 7 * 10                        # 4 sites per a/sqrt2 x a x a/sqrt2 repeat
+    p = x[:, 1:4]
+    d, _ = cKDTree(p).query(p, k=5)
+    inner = np.all((p > 3) & (p < p.max(axis=0) - 3), axis=1)
+    np.testing.assert_allclose(d[inner, 1:], a * np.sqrt(3) / 4, rtol=1e-9)   # 4 bonds of a*sqrt(3)/4
+
+
+def test_amorphous_density_separation_and_seed():
+    from scipy.spatial import cKDTree
+    from totem_amd import specimen
+    x = specimen.amorphous(30.0, 30.0, 5.0, 25.0, seed=7)
+    assert len(x) == round(0.1003 * 30 * 30 * 20)
+    p = x[:, 1:4]
+    assert p[:, 2].min() >= 5.0 and p[:, 2].max() < 25.0 and p[:, :2].min() >= 0 and p[:, :2].max() < 30.0
+    d, _ = cKDTree(p, boxsize=[30.0, 30.0, 1e3]).query(p, k=2)
+    assert d[:, 1].min() >= 1.4
+    np.testing.assert_array_equal(x, specimen.amorphous(30.0, 30.0, 5.0, 25.0, seed=7))
+    assert not np.array_equal(x, specimen.amorphous(30.0, 30.0, 5.0, 25.0, seed=8))

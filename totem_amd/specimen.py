@@ -71,9 +71,12 @@ def diamond_110(lx, ly, lz, a=5.431, Z=14, B=0.0, origin=(0.0, 0.0, 0.0)):
     sh = np.array([[i, j, k] for i in range(-2, 3) for j in range(-2, 3) for k in range(-2, 3)])
     pts = (cart[None, :, :] + sh[:, None, :] * a).reshape(-1, 3)
     r = np.stack([pts @ ex, pts @ ey, pts @ ez], axis=1)
-    r = r % np.array([ax, ay, az])
-    r = np.unique(np.round(r, 9), axis=0)
-    nx, ny, nz = int(np.ceil(lx / ax)), int(np.ceil(ly / ay)), int(np.ceil(lz / az))
+    per = np.array([ax, ay, az])
+    r = np.round(r % per, 9)
+    r[np.abs(r - per) < 1e-6] = 0.0  # a coordinate that rounds onto the period is the same site as 0
+    r = np.unique(r, axis=0)
+    assert len(r) == 4, "diamond [110] repeat must hold 4 sites"
+    nx, ny, nz = (int(np.ceil(l / p - 1e-9)) for l, p in ((lx, ax), (ly, ay), (lz, az)))
     i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     cells = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1) * np.array([ax, ay, az])
     pos = (cells[:, None, :] + r[None, :, :]).reshape(-1, 3) + np.asarray(origin)
@@ -85,45 +88,39 @@ def diamond_110(lx, ly, lz, a=5.431, Z=14, B=0.0, origin=(0.0, 0.0, 0.0)):
     return x
 
 
-def amorphous(lx, ly, z0, z1, density_per_A3=0.1003, dmin=1.4, Z=6, seed=0, B=0.0):
-    """Uniform random atoms in [0,lx)x[0,ly)x[z0,z1) with a minimum separation (periodic in x,y),
-    by sequential rejection on a cell list.  Seeded (numpy PCG64)."""
+def amorphous(lx, ly, z0, z1, density_per_A3=0.1003, dmin=1.4, Z=6, seed=0, B=0.0, batch=8192):
+    """Uniform random atoms in [0,lx)x[0,ly)x[z0,z1) at the given number density with a minimum
+    separation dmin (periodic in x and y), by random sequential addition: candidates are drawn in
+    batches, dropped if closer than dmin to an accepted atom, and accepted in draw order within a
+    batch (a candidate loses to an earlier accepted one).  Seeded (numpy PCG64)."""
+    from scipy.spatial import cKDTree
     rng = np.random.Generator(np.random.PCG64(seed))
-    target = int(round(density_per_A3 * lx * ly * (z1 - z0)))
-    cs = dmin
-    ncx, ncy, ncz = max(1, int(lx // cs)), max(1, int(ly // cs)), max(1, int((z1 - z0) // cs))
-    grid = {}
-    pts = []
-    tries = 0
-    d2 = dmin * dmin
-    while len(pts) < target and tries < target * 200:
-        tries += 1
-        p = rng.random(3) * np.array([lx, ly, z1 - z0])
-        ci = (int(p[0] / lx * ncx) % ncx, int(p[1] / ly * ncy) % ncy, min(int(p[2] / (z1 - z0) * ncz), ncz - 1))
-        ok = True
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    kz = ci[2] + dz
-                    if kz < 0 or kz >= ncz:
-                        continue
-                    for q in grid.get(((ci[0] + dx) % ncx, (ci[1] + dy) % ncy, kz), ()):
-                        dd = p - q
-                        dd[0] -= lx * np.round(dd[0] / lx)
-                        dd[1] -= ly * np.round(dd[1] / ly)
-                        if dd @ dd < d2:
-                            ok = False
-                            break
-                    if not ok:
-                        break
-                if not ok:
-                    break
-            if not ok:
-                break
-        if ok:
-            grid.setdefault(ci, []).append(p)
-            pts.append(p)
-    pos = np.asarray(pts) + np.array([0.0, 0.0, z0])
+    lz = z1 - z0
+    target = int(round(density_per_A3 * lx * ly * lz))
+    box = np.array([lx, ly, 2.0 * lz + 4.0 * dmin])  # z padded: no wrap-around within dmin
+    scale = np.array([lx, ly, lz])
+    pts = np.zeros((0, 3))
+    rounds = 0
+    while len(pts) < target:
+        rounds += 1
+        if rounds > 10000:
+            raise RuntimeError("amorphous: density too high for dmin")
+        cand = rng.random((batch, 3)) * scale
+        if len(pts):
+            d, _ = cKDTree(pts, boxsize=box).query(cand, k=1, distance_upper_bound=dmin)
+            cand = cand[~(d < dmin)]
+        if len(cand) == 0:
+            continue
+        pairs = cKDTree(cand, boxsize=box).query_pairs(dmin, output_type="ndarray")
+        keep = np.ones(len(cand), dtype=bool)
+        if len(pairs):
+            pairs = np.sort(pairs, axis=1)
+            pairs = pairs[np.lexsort((pairs[:, 0], pairs[:, 1]))]
+            for i, j in pairs:
+                if keep[i]:
+                    keep[j] = False
+        pts = np.concatenate([pts, cand[keep][:target - len(pts)]])
+    pos = pts + np.array([0.0, 0.0, z0])
     x = np.zeros((len(pos), 8))
     x[:, 0] = Z
     x[:, 1:4] = pos

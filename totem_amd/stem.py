@@ -201,3 +201,28 @@ def perovskite_c2(scan=64, step=3):
 
 
 C2_PHONON_SIGMA = {38: 0.08, 22: 0.08, 8: 0.10}
+
+
+def precision_c3(scan=32, side=100.0, thick=300.0, cap=50.0, n=2048):
+    """The precision-stress workload: diamond-cubic Si (a = 5.431 A) viewed along [110], about
+    ``side`` x ``side`` x ``thick`` A (whole repeats), capped top and bottom by ``cap`` A of amorphous
+    carbon at 2.0 g/cm^3 (0.1003 atoms/A^3) with 1.4 A minimum separation (seeded), on an n^2 grid,
+    slices of 1.92 A, 200 kV, 22 mrad, Lobato, 32 x 32 scan over one [110] repeat (periodic windows)."""
+    a = 5.431
+    ax, ay = a / np.sqrt(2), a
+    nx = int(np.ceil(side / ax))
+    ny = int(round(nx * ax / ay))
+    box = nx * ax
+    si = specimen.diamond_110(nx * ax, ny * ay, thick, a=a, origin=(0.0, 0.0, cap))
+    si = si[(si[:, 1] < box) & (si[:, 2] < box)]
+    z_si = si[:, 3].max() + 0.5
+    top = specimen.amorphous(box, box, 0.0, cap, Z=6, seed=1003)
+    bottom = specimen.amorphous(box, box, z_si, z_si + cap, Z=6, seed=2003)
+    atoms = np.concatenate([top, si, bottom])
+    dz = 1.92
+    S = int(np.ceil((z_si + cap) / dz))
+    dx = box / n
+    step = max(1, int(round(ax / dx / scan)))
+    return StemConfig(atoms=atoms, np_side=n, dx=dx, n=n, kv=200.0, alpha_mrad=22.0, slice_thickness=dz,
+                      n_slices=S, slice_origin=-dz / 4, paraflag="l", detectors_mrad=((60.0, 200.0),),
+                      width_red=scan, hight_red=scan, step=step, begin_row=1, begin_col=1)
