@@ -177,6 +177,29 @@ int totem_server_poll(totem_server *srv, long long *ids, int cap, int *n_done);
 int totem_server_sync(totem_server *srv);
 int totem_server_destroy(totem_server *srv);
 
+/* ------------------------------------------------------- batch HRTEM imaging
+ * Replaces the computation of the reference's Python tool (cuda_stem64.py:164-704,
+ * class stem: createNewPP2, WholeTCC2D_newTEM_forpar, the product with the exit
+ * wave, ifftshift, batched inverse FFT, |.|^2 and the gfsf weighting).  The random
+ * draws come from the caller (the reference draws them with curand, :667). */
+typedef struct totem_hrtem_desc {
+  int n;                 /* grid side N */
+  int batches;           /* B images */
+  int mtotal;            /* focal members (only member 0 reaches the output, as in the reference) */
+  double lambda_nm;      /* para_part1['lambda'] */
+  double alafa;          /* para_part1['alafa'], convergence semi-angle of the source (rad) */
+  double delta_yita;     /* para_part1['delta_yita'] */
+  double gfsf;           /* para_part1['gfsf'] (weight of the single accumulated member) */
+  const double *u, *v;   /* [N][N] reciprocal grids U, V (1/nm) */
+  const double *wave;    /* [N][N] complex128 as (re, im) pairs: centred reciprocal-space exit wave */
+  const double *pp2;     /* [49] nominal parameters (para_part2) */
+  const double *mrr;     /* [21] draw ranges (myrandrange) */
+  const double *ai;      /* [B][21] uniform draws in [0, 1) */
+} totem_hrtem_desc;
+/* images: host [B][N][N] doubles; paras (may be NULL): host [B][49], the drawn
+ * parameter rows (newPp2).  Synchronous on `stream` (NULL: the library's). */
+int totem_hrtem_images(const totem_hrtem_desc *desc, double *paras, double *images, void *stream);
+
 /* Batched in-place 2-D FFT of `batch` n x n complex64 matrices on the device
  * (sign -1 forward, +1 inverse, unnormalised).  Test/diagnostic entry point. */
 int totem_fft2d(void *data, int n, int batch, int sign, void *stream);

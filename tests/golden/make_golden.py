@@ -12,6 +12,11 @@ series), both on a 256^2 potential over 36 A with a 128^2 probe window:
   and 40 (complex64) plus, for every slice, sum V and sum |V|^2 (FP64);
 * the mex outputs of a 4 x 4 scan: ``myresul`` [D, A] and ``mid_ceng_mat`` [D*A, L].
 
+A second set comes from the reference's Python-tool input file ``pydata.mat`` (read as data): its
+parameter structs, U/V grids and exit wave (320^2) go to ``pydata_inputs.npz``; the FP64 oracle's
+batch HRTEM images for seeded draws (``HRTEM_CASES``) go to ``pydata_golden.npz`` as the crops the
+tool's consumer cuts (``position`` from ``othervar``) plus per-image sums.
+
 Only this script reads the reference checkout; the tests read the ``.npz`` files.
 """
 import os
@@ -72,8 +77,59 @@ def summarise(V, res, mid):
             "myresul": res, "mid": mid}
 
 
+REF_PY = "/root/reference/pydata.mat"
+HRTEM_CASES = {"pydata": dict(seed=1006, batches=3),
+               "focal3": dict(seed=1007, batches=2, mtotal=3, delta_yita=4.0, gfsf=0.6)}
+
+
+def hrtem_case(inputs, name):
+    """(para_part1, ai [B, 21]) of golden HRTEM case ``name`` on the stored pydata inputs."""
+    c = dict(HRTEM_CASES[name])
+    rng = np.random.Generator(np.random.PCG64(c.pop("seed")))
+    ai = rng.random((c.pop("batches"), 21))
+    pp1 = {k: float(inputs["pp1_" + k]) for k in ("sampling", "mtotal", "alafa", "lambda", "gmax", "gfsf",
+                                                   "delta_yita")}
+    pp1.update({k: float(v) for k, v in c.items()})
+    return pp1, ai
+
+
+def crop(inputs, img):
+    ov = inputs["othervar"]
+    top, left, width, height = int(ov[3]), int(ov[4]), int(ov[5]), int(ov[6])
+    return img[..., left:left + width, top:top + height]
+
+
+def hrtem_golden(inputs):
+    sys.path.insert(0, ROOT)
+    from oracle import hrtem as oh
+    out = {}
+    for name in HRTEM_CASES:
+        pp1, ai = hrtem_case(inputs, name)
+        paras = oh.new_pp2(inputs["pp2"], inputs["mrr"], ai)
+        img = oh.images(inputs["mywave"], paras, inputs["U"], inputs["V"], pp1["lambda"], pp1["alafa"],
+                        int(pp1["mtotal"]), pp1["delta_yita"], pp1["gfsf"])
+        out[name + "_crop"] = crop(inputs, img)
+        out[name + "_sum"] = img.sum(axis=(1, 2))
+        out[name + "_paras"] = paras
+    return out
+
+
 def main():
     import scipy.io as sio
+    m = sio.loadmat(REF_PY)
+    p1 = m["para_part1"][0][0]
+    names = ("sampling", "mtotal", "alafa", "lambda", "gmax", "gfsf", "delta_yita", "tilt", "phitilt", "tiltx",
+             "tilty")
+    inputs = {"pp1_" + k: np.float64(np.asarray(p1[i]).ravel()[0]) for i, k in enumerate(names)}
+    inputs["pp2"] = np.array([float(np.asarray(x).ravel()[0]) for x in m["para_part2"][0][0]])
+    inputs["mrr"] = np.array([float(np.asarray(x).ravel()[0]) for x in m["myrandrange"][0][0]])
+    inputs["U"] = np.asarray(m["U"], dtype=np.float64)
+    inputs["V"] = np.asarray(m["V"], dtype=np.float64)
+    inputs["mywave"] = np.asarray(m["mywave"], dtype=np.complex128)
+    inputs["othervar"] = np.asarray(m["othervar"], dtype=np.float64).ravel()
+    inputs["commpara"] = np.asarray(m["commpara"], dtype=np.float64).ravel()
+    np.savez_compressed(os.path.join(HERE, "pydata_inputs.npz"), **inputs)
+    np.savez_compressed(os.path.join(HERE, "pydata_golden.npz"), **hrtem_golden(inputs))
     d = sio.loadmat(REF)["allresul"][0, 0]
     atoms = np.asarray(d["x_new"], dtype=np.float64)
     faces = np.asarray(d["slicethick"], dtype=np.float64).ravel()

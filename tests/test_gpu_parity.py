@@ -31,7 +31,7 @@ def _rel_l2(a, b):
 
 # ---------------------------------------------------------------------------- FFT
 
-@pytest.mark.parametrize("n", [64, 128, 256, 512, 768, 1024, 1536, 2048])
+@pytest.mark.parametrize("n", [64, 128, 256, 320, 512, 768, 1024, 1536, 2048])
 def test_fft2d_matches_numpy(engine, n):
     torch = _torch()
     print("case fft2d n=%d" % n, flush=True)

@@ -176,7 +176,7 @@ def test_library_loads_without_gpu():
     L = ctypes.CDLL(engine.CORE)
     L.totem_abi_version.restype = ctypes.c_int
     assert L.totem_abi_version() == engine.ABI_VERSION
-    for n in (64, 128, 256, 512, 768, 1024, 1536, 2048):
+    for n in (64, 128, 256, 320, 512, 768, 1024, 1536, 2048):
         assert L.totem_fft_size_supported(n) == 1
     assert L.totem_fft_size_supported(1000) == 0
 

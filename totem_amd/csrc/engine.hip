@@ -83,7 +83,7 @@ const float2* tw_table(int n) {
   return static_cast<const float2*>(p);
 }
 
-static hipStream_t lib_stream() {
+hipStream_t lib_stream() {
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
   Context& c = ctx();
@@ -805,6 +805,14 @@ int totem_build_transmission(const totem_potential_desc* desc, void* t_dev, doub
     require(desc && t_dev, "desc and t_dev are required");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : lib_stream();
     build_transmission_impl(*desc, static_cast<float2*>(t_dev), v_re, v_im, st);
+  });
+}
+
+int totem_hrtem_images(const totem_hrtem_desc* desc, double* paras, double* images, void* stream) {
+  return guarded([&] {
+    require(desc != nullptr, "desc is required");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : totem::lib_stream();
+    totem::hrtem_images_impl(*desc, paras, images, st);
   });
 }
 

@@ -333,6 +333,7 @@ template <int N> struct Plan;
 template <> struct Plan<64> : PlanDef<64, 16, 4, 4, 4> {};
 template <> struct Plan<128> : PlanDef<128, 16, 8, 4, 4> {};
 template <> struct Plan<256> : PlanDef<256, 16, 16, 16> {};
+template <> struct Plan<320> : PlanDef<320, 16, 4, 4, 4, 5> {};
 template <> struct Plan<512> : PlanDef<512, 32, 16, 16, 2> {};
 template <> struct Plan<768> : PlanDef<768, 32, 8, 3, 8, 4> {};
 template <> struct Plan<1024> : PlanDef<1024, 64, 16, 16, 4> {};

@@ -79,6 +79,10 @@ void validate_job(const totem_stem_job& j);
 void mid_walk(const totem_stem_job& j, std::vector<int>& mos, std::vector<int>& hit);
 // Probes in flight whose waves fit the memory-side cache (the default batch).
 int preferred_batch(int n);
+// The library's own stream on the current device (used when the caller passes none).
+hipStream_t lib_stream();
+// Batch HRTEM imaging (hrtem.hip).
+void hrtem_images_impl(const totem_hrtem_desc& d, double* paras_out, double* images, hipStream_t st);
 // Window origin (row, column) of scan position d (periodic when j.wrap).
 void window_origin(const totem_stem_job& j, int d, int& r0, int& c0);
 

@@ -648,7 +648,7 @@ template <int N> void twiddle_host_n(std::vector<float2>& out) {
   }
 }
 
-#define TOTEM_FOR_SIZES(X) X(64) X(128) X(256) X(512) X(768) X(1024) X(1536) X(2048)
+#define TOTEM_FOR_SIZES(X) X(64) X(128) X(256) X(320) X(512) X(768) X(1024) X(1536) X(2048)
 
 void twiddle_host(int n, std::vector<float2>& out) {
   out.clear();

@@ -35,7 +35,7 @@ EXPORTS = (
     "totem_project_lobato", "totem_project_peng", "totem_potential_lobato", "totem_potential_peng",
     "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
     "totem_stem_workspace_bytes", "totem_stem_run", "totem_kernel_times", "totem_fft2d",
-    "totem_stem_preferred_batch", "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
+    "totem_stem_preferred_batch", "totem_hrtem_images", "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
     "totem_server_destroy",
 )
 
@@ -83,6 +83,15 @@ class StemJob(ctypes.Structure):
     ]
 
 
+class HrtemDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", _I), ("batches", _I), ("mtotal", _I),
+        ("lambda_nm", ctypes.c_double), ("alafa", ctypes.c_double), ("delta_yita", ctypes.c_double),
+        ("gfsf", ctypes.c_double),
+        ("u", _DP), ("v", _DP), ("wave", _DP), ("pp2", _DP), ("mrr", _DP), ("ai", _DP),
+    ]
+
+
 _lib = None
 
 
@@ -116,6 +125,8 @@ def lib():
         L.totem_kernel_times.restype = None
         L.totem_fft2d.argtypes = [_VP, _I, _I, _I, _VP]
         L.totem_fft2d.restype = _I
+        L.totem_hrtem_images.argtypes = [ctypes.POINTER(HrtemDesc), _DP, _DP, _VP]
+        L.totem_hrtem_images.restype = _I
         L.totem_stem_preferred_batch.argtypes = [_I]
         L.totem_stem_preferred_batch.restype = _I
         L.totem_server_create.argtypes = [ctypes.c_size_t, _I, _I, ctypes.POINTER(_VP)]
@@ -281,6 +292,27 @@ def stem_job(n, np_side, slices, t_ptr, probes_ptr, n_probes, prop_ptr, aper_ptr
 def preferred_batch(n):
     """Probes in flight whose n x n waves fit the GPU's memory-side cache."""
     return int(lib().totem_stem_preferred_batch(int(n)))
+
+
+def hrtem_images(n, lambda_nm, alafa, delta_yita, gfsf, mtotal, U, V, mywave, pp2, mrr, ai):
+    """Batch HRTEM images (B = len(ai)) and the drawn parameter rows: (images [B, N, N], paras [B, 49])."""
+    ai = np.ascontiguousarray(np.atleast_2d(ai), dtype=np.float64)
+    B = ai.shape[0]
+    wave = np.ascontiguousarray(np.asarray(mywave, dtype=np.complex128)).view(np.float64)
+    keep = [_f64(U), _f64(V), wave, _f64(pp2), _f64(mrr), ai]
+    assert keep[0].shape == (n, n) and keep[1].shape == (n, n) and wave.shape == (n, 2 * n)
+    assert keep[3].size == 49 and keep[4].size >= 21 and ai.shape[1] == 21
+    d = HrtemDesc()
+    d.n, d.batches, d.mtotal = int(n), int(B), int(mtotal)
+    d.lambda_nm, d.alafa, d.delta_yita, d.gfsf = float(lambda_nm), float(alafa), float(delta_yita), float(gfsf)
+    d.u, d.v, d.wave, d.pp2, d.ai = _dp(keep[0]), _dp(keep[1]), _dp(wave), _dp(keep[3]), _dp(ai)
+    mrr = np.ascontiguousarray(keep[4][:21])
+    keep.append(mrr)
+    d.mrr = _dp(mrr)
+    images = np.zeros((B, n, n))
+    paras = np.zeros((B, 49))
+    check(lib().totem_hrtem_images(ctypes.byref(d), _dp(paras), _dp(images), None))
+    return images, paras
 
 
 def stem_workspace_bytes(job, batch):
