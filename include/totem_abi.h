@@ -200,6 +200,19 @@ typedef struct totem_hrtem_desc {
  * parameter rows (newPp2).  Synchronous on `stream` (NULL: the library's). */
 int totem_hrtem_images(const totem_hrtem_desc *desc, double *paras, double *images, void *stream);
 
+/* HRTEM image of an exit wave (ToTEM_submit.m:1327-1335):
+ * image = sum_m gfsf[m] |IFFT2(ifftshift(wave * tcc[m] * ap))|^2 with wave the centred
+ * reciprocal-space exit wave (complex128 [n][n] as (re, im) pairs), tcc the centred
+ * transfer functions (complex128 [m][n][n]), ap the objective aperture [n][n]; all host
+ * arrays, image host [n][n].  Synchronous. */
+int totem_tcc_image(int n, int m, const double *wave, const double *tcc, const double *ap, const double *gfsf,
+                    double *image, void *stream);
+/* out = sum_m weights[m] |fields[m]|^2 over device complex64 fields [m][n][n], quadrant-swapped
+ * (fftshift) when shift != 0; out host [n][n] (the CBED pattern of ToTEM_submit.m:1459-1472 from
+ * a job's exit waves).  Synchronous. */
+int totem_weighted_pattern(int n, int m, const void *fields, const double *weights, int shift, double *out,
+                           void *stream);
+
 /* Batched in-place 2-D FFT of `batch` n x n complex64 matrices on the device
  * (sign -1 forward, +1 inverse, unnormalised).  Test/diagnostic entry point. */
 int totem_fft2d(void *data, int n, int batch, int sign, void *stream);

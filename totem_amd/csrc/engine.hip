@@ -816,6 +816,22 @@ int totem_hrtem_images(const totem_hrtem_desc* desc, double* paras, double* imag
   });
 }
 
+int totem_tcc_image(int n, int m, const double* wave, const double* tcc, const double* ap, const double* gfsf,
+                    double* image, void* stream) {
+  return guarded([&] {
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : totem::lib_stream();
+    totem::tcc_image_impl(n, m, wave, tcc, ap, gfsf, image, st);
+  });
+}
+
+int totem_weighted_pattern(int n, int m, const void* fields, const double* weights, int shift, double* out,
+                           void* stream) {
+  return guarded([&] {
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : totem::lib_stream();
+    totem::weighted_pattern_impl(n, m, fields, weights, shift, out, st);
+  });
+}
+
 int totem_stem_preferred_batch(int n) { return n > 0 ? totem::preferred_batch(n) : 0; }
 
 size_t totem_stem_workspace_bytes(const totem_stem_job* job, int batch) {

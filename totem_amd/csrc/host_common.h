@@ -81,8 +81,12 @@ void mid_walk(const totem_stem_job& j, std::vector<int>& mos, std::vector<int>& 
 int preferred_batch(int n);
 // The library's own stream on the current device (used when the caller passes none).
 hipStream_t lib_stream();
-// Batch HRTEM imaging (hrtem.hip).
+// Batch HRTEM imaging, transfer-function imaging and weighted patterns (hrtem.hip).
 void hrtem_images_impl(const totem_hrtem_desc& d, double* paras_out, double* images, hipStream_t st);
+void tcc_image_impl(int N, int M, const double* wave, const double* tcc, const double* ap, const double* gfsf,
+                    double* image, hipStream_t st);
+void weighted_pattern_impl(int N, int M, const void* fields, const double* weights, int shift, double* out,
+                           hipStream_t st);
 // Window origin (row, column) of scan position d (periodic when j.wrap).
 void window_origin(const totem_stem_job& j, int d, int& r0, int& c0);
 

@@ -150,6 +150,45 @@ __global__ void __launch_bounds__(256) k_hrtem_intensity(const float2* __restric
   out[i] = gfsf * (a * a + b * b);
 }
 
+// Weighted sum of the intensities of M fields [M][N][N] (complex64), optionally
+// quadrant-swapped on output (fftshift): out[r][c] = sum_m w[m] |x_m(src)|^2.
+__global__ void __launch_bounds__(256) k_weighted_intensity(const float2* __restrict__ x, int N, int M,
+                                                            const double* __restrict__ w, int shift,
+                                                            double* __restrict__ out) {
+  const long NN = (long)N * N;
+  const long pix = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= NN) return;
+  long src = pix;
+  if (shift) {
+    const int r = (int)(pix / N), c = (int)(pix - (long)r * N);
+    const int sr = r + N / 2 >= N ? r - N / 2 : r + N / 2, sc = c + N / 2 >= N ? c - N / 2 : c + N / 2;
+    src = (long)sr * N + sc;
+  }
+  double s = 0.0;
+  for (int m = 0; m < M; ++m) {
+    const float2 v = x[m * NN + src];
+    s += w[m] * ((double)v.x * v.x + (double)v.y * v.y);
+  }
+  out[pix] = s;
+}
+
+// out[m][swap(r, c)] = wave[r][c] * tcc[m][r][c] * ap[r][c] (FP64 product, complex64 out)
+__global__ void __launch_bounds__(256) k_tcc_field(const double2* __restrict__ wave, const double2* __restrict__ tcc,
+                                                   const double* __restrict__ ap, int N, int M,
+                                                   float2* __restrict__ out) {
+  const long NN = (long)N * N;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= NN * M) return;
+  const int m = (int)(idx / NN);
+  const long pix = idx - (long)m * NN;
+  const int r = (int)(pix / N), c = (int)(pix - (long)r * N);
+  const double2 a = wave[pix], b = tcc[idx];
+  const double g = ap[pix];
+  const double re = (a.x * b.x - a.y * b.y) * g, im = (a.x * b.y + a.y * b.x) * g;
+  const int dr = r + N / 2 >= N ? r - N / 2 : r + N / 2, dc = c + N / 2 >= N ? c - N / 2 : c + N / 2;
+  out[(long)m * NN + (long)dr * N + dc] = make_float2((float)re, (float)im);
+}
+
 struct HrtemScratch {
   std::mutex mu;
   DevBuf wave, u, v, pp2, mrr, ai, paras, field, img;
@@ -159,7 +198,59 @@ void reserve_buf(DevBuf& b, size_t bytes) {
   if (b.bytes < bytes) b.alloc(bytes);
 }
 
+HrtemScratch& scratch() {
+  static HrtemScratch sc;
+  return sc;
+}
+
 }  // namespace
+
+void tcc_image_impl(int N, int M, const double* wave, const double* tcc, const double* ap, const double* gfsf,
+                    double* image, hipStream_t st) {
+  require(N > 0 && N % 2 == 0 && M > 0, "n must be positive and even, m positive");
+  require(size_supported(N), "grid " + std::to_string(N) + " has no compiled FFT plan", TOTEM_ERR_UNSUPPORTED);
+  require(wave && tcc && ap && gfsf && image, "wave, tcc, ap, gfsf and image are required");
+  HrtemScratch& sc = scratch();
+  std::lock_guard<std::mutex> lk(sc.mu);
+  const size_t NN = (size_t)N * N;
+  reserve_buf(sc.wave, NN * sizeof(double2));
+  reserve_buf(sc.u, NN * sizeof(double));              // aperture
+  reserve_buf(sc.paras, (size_t)M * NN * sizeof(double2));  // tcc stack
+  reserve_buf(sc.pp2, (size_t)M * sizeof(double));     // weights
+  reserve_buf(sc.field, (size_t)M * NN * sizeof(float2));
+  reserve_buf(sc.img, NN * sizeof(double));
+  HIP_OK(hipMemcpyAsync(sc.wave.p, wave, NN * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(sc.paras.p, tcc, (size_t)M * NN * sizeof(double2), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(sc.u.p, ap, NN * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(sc.pp2.p, gfsf, (size_t)M * sizeof(double), hipMemcpyHostToDevice, st));
+  const long total = (long)M * (long)NN;
+  hipLaunchKernelGGL(k_tcc_field, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, sc.wave.as<double2>(),
+                     sc.paras.as<double2>(), sc.u.as<double>(), N, M, sc.field.as<float2>());
+  fft2d(N, st, sc.field.as<float2>(), sc.field.as<float2>(), M, +1, 2, 0.f);
+  hipLaunchKernelGGL(k_weighted_intensity, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, st,
+                     sc.field.as<float2>(), N, M, sc.pp2.as<double>(), 0, sc.img.as<double>());
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(image, sc.img.p, NN * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+}
+
+void weighted_pattern_impl(int N, int M, const void* fields, const double* weights, int shift, double* out,
+                           hipStream_t st) {
+  require(N > 0 && N % 2 == 0 && M > 0, "n must be positive and even, m positive");
+  require(fields && weights && out, "fields, weights and out are required");
+  HrtemScratch& sc = scratch();
+  std::lock_guard<std::mutex> lk(sc.mu);
+  const size_t NN = (size_t)N * N;
+  reserve_buf(sc.pp2, (size_t)M * sizeof(double));
+  reserve_buf(sc.img, NN * sizeof(double));
+  HIP_OK(hipMemcpyAsync(sc.pp2.p, weights, (size_t)M * sizeof(double), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_weighted_intensity, dim3((unsigned)((NN + 255) / 256)), dim3(256), 0, st,
+                     static_cast<const float2*>(fields), N, M, sc.pp2.as<double>(), shift ? 1 : 0,
+                     sc.img.as<double>());
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, sc.img.p, NN * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+}
 
 void hrtem_images_impl(const totem_hrtem_desc& d, double* paras_out, double* images, hipStream_t st) {
   const int N = d.n, B = d.batches;
@@ -168,7 +259,7 @@ void hrtem_images_impl(const totem_hrtem_desc& d, double* paras_out, double* ima
   require(size_supported(N), "grid " + std::to_string(N) + " has no compiled FFT plan", TOTEM_ERR_UNSUPPORTED);
   require(d.mtotal >= 1, "mtotal must be >= 1");
   require(d.u && d.v && d.wave && d.pp2 && d.mrr && d.ai && images, "u, v, wave, pp2, mrr, ai and images are required");
-  static HrtemScratch sc;
+  HrtemScratch& sc = scratch();
   std::lock_guard<std::mutex> lk(sc.mu);
   const size_t NN = (size_t)N * N;
   reserve_buf(sc.wave, NN * sizeof(double2));

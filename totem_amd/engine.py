@@ -35,7 +35,8 @@ EXPORTS = (
     "totem_project_lobato", "totem_project_peng", "totem_potential_lobato", "totem_potential_peng",
     "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
     "totem_stem_workspace_bytes", "totem_stem_run", "totem_kernel_times", "totem_fft2d",
-    "totem_stem_preferred_batch", "totem_hrtem_images", "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
+    "totem_stem_preferred_batch", "totem_hrtem_images", "totem_tcc_image", "totem_weighted_pattern",
+    "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
     "totem_server_destroy",
 )
 
@@ -127,6 +128,10 @@ def lib():
         L.totem_fft2d.restype = _I
         L.totem_hrtem_images.argtypes = [ctypes.POINTER(HrtemDesc), _DP, _DP, _VP]
         L.totem_hrtem_images.restype = _I
+        L.totem_tcc_image.argtypes = [_I, _I, _DP, _DP, _DP, _DP, _DP, _VP]
+        L.totem_tcc_image.restype = _I
+        L.totem_weighted_pattern.argtypes = [_I, _I, _VP, _DP, _I, _DP, _VP]
+        L.totem_weighted_pattern.restype = _I
         L.totem_stem_preferred_batch.argtypes = [_I]
         L.totem_stem_preferred_batch.restype = _I
         L.totem_server_create.argtypes = [ctypes.c_size_t, _I, _I, ctypes.POINTER(_VP)]
@@ -313,6 +318,28 @@ def hrtem_images(n, lambda_nm, alafa, delta_yita, gfsf, mtotal, U, V, mywave, pp
     paras = np.zeros((B, 49))
     check(lib().totem_hrtem_images(ctypes.byref(d), _dp(paras), _dp(images), None))
     return images, paras
+
+
+def tcc_image(wave, tcc, ap, gfsf):
+    """sum_m gfsf[m] |IFFT2(ifftshift(wave * tcc[m] * ap))|^2 (host arrays, centred inputs)."""
+    wave = np.ascontiguousarray(np.asarray(wave, dtype=np.complex128))
+    n = wave.shape[0]
+    tcc = np.ascontiguousarray(np.asarray(tcc, dtype=np.complex128).reshape(-1, n, n))
+    ap = _f64(ap)
+    gfsf = _f64(gfsf)
+    assert wave.shape == (n, n) and ap.shape == (n, n) and gfsf.size == tcc.shape[0]
+    out = np.zeros((n, n))
+    check(lib().totem_tcc_image(n, tcc.shape[0], _dp(wave.view(np.float64)), _dp(tcc.reshape(-1).view(np.float64)),
+                                _dp(ap), _dp(gfsf), _dp(out), None))
+    return out
+
+
+def weighted_pattern(fields_ptr, n, weights, shift=True):
+    """sum_m weights[m] |fields[m]|^2 of device complex64 fields [m, n, n] (fftshifted when shift)."""
+    w = _f64(weights)
+    out = np.zeros((n, n))
+    check(lib().totem_weighted_pattern(int(n), int(w.size), _VP(fields_ptr), _dp(w), int(bool(shift)), _dp(out), None))
+    return out
 
 
 def stem_workspace_bytes(job, batch):

@@ -68,11 +68,12 @@ def reciprocal_grids(np_side, dx):
     return gx, gy, s2
 
 
-def propagator(n, dx, kv, dz, tilt_mrad=0.0, tilt_phi_deg=0.0):
-    """Fresnel propagator H(k) = exp(-i pi lambda dz |k+tilt|^2), cut at |k+tilt|^2 > 1/(9 dx^2),
+def propagator(n, dx, kv, dz, tilt_mrad=0.0, tilt_phi_deg=0.0, cut=9.0):
+    """Fresnel propagator H(k) = exp(-i pi lambda dz |k+tilt|^2), zero where |k+tilt|^2 > 1/(cut dx^2),
     returned ifftshifted (zero frequency at [0, 0]), complex128 [n, n].
 
-    ``ToTEM_submit.m:1520-1533``.
+    STEM and CBED cut at 2/3 of the Nyquist radius (cut 9, ``ToTEM_submit.m:1520-1533,1371-1374``);
+    the HRTEM plane-wave path at 1/2 (cut 16, ``:1311-1314``).
     """
     lam_nm = wavelength_nm(kv)
     tilt = tilt_mrad * 1e-3
@@ -83,7 +84,7 @@ def propagator(n, dx, kv, dz, tilt_mrad=0.0, tilt_phi_deg=0.0):
     px, py = np.meshgrid(f, f)
     p2 = (px + tiltx * 0.1) ** 2 + (py + tilty * 0.1) ** 2
     h = np.exp(-1j * np.pi * lam_nm * 10 * dz * p2)
-    h[p2 > 1.0 / (9 * dx * dx)] = 0
+    h[p2 > 1.0 / (cut * dx * dx)] = 0
     return np.fft.ifftshift(h)
 
 
@@ -209,14 +210,12 @@ def _chi_gradient(ab, u, v, lam):
     return gu, gv
 
 
-def probe_stack(n, dx, kv, alpha_mrad, ab=None, mtotal=1, focal_rms_nm=0.0, source_mrad=0.0,
-                tilt_mrad=0.0, tilt_phi_deg=0.0):
-    """Probe wavefunctions [M, n, n] complex128 and focal-series weights gfsf [M].
-
-    probe_m = fftshift(ifft2(ifftshift(ap * tcc_m))) / sqrt(sum |ifft2(...)|^2)
-    (``ToTEM_submit.m:788-806``) with tcc_m = E_s * E_f,m * exp(-i 2 pi W / lambda)
-    (``WholeTCC2D_newTEM:2274-2375``).  Sampling in nm = dx*0.1 (``CommonPara_TEM:2128``).
-    """
+def tcc_stack(n, dx, kv, alpha_mrad, ab=None, mtotal=1, focal_rms_nm=0.0, source_mrad=0.0,
+              tilt_mrad=0.0, tilt_phi_deg=0.0):
+    """Transfer functions of the focal series, centred: (tcc [M, n, n] complex128, gfsf [M],
+    aperture [n, n]) with tcc_m = E_s * E_f,m * exp(-i 2 pi W / lambda) (``WholeTCC2D_newTEM``,
+    ``ToTEM_submit.m:2274-2375``) and the aperture |k| <= alpha/lambda (``myaperture`` with flag 1,
+    ``:715-736``).  Sampling in nm = dx*0.1 (``CommonPara_TEM:2128``)."""
     ab = ab or Aberrations()
     lam = wavelength_nm(kv)
     sampling = dx * 0.1
@@ -242,18 +241,30 @@ def probe_stack(n, dx, kv, alpha_mrad, ab=None, mtotal=1, focal_rms_nm=0.0, sour
     kk = np.sqrt(uu * uu + vv * vv)
     ap = np.ones((n, n))
     ap[(kk < 0) | (kk > gmax * 0.01 / lam)] = 0
-    probes = []
     M = len(gfsf)
+    tcc = np.zeros((M, n, n), dtype=np.complex128)
     for m in range(1, M + 1):
         gu = gu0 + ab.C1 * lam * U
         gv = gv0 + ab.C1 * lam * V
         Es = np.exp(-alafa * alafa * np.pi * np.pi / lam / lam * (gu ** 2 + gv ** 2))
         Ef = np.exp(-1j * np.pi * (m - (M + 1) / 2) * delta_yita * (U ** 2 + V ** 2) * lam)
-        tcc = Es * Ef * np.exp(-1j * 2 * np.pi * W / lam)
-        w = np.fft.ifft2(np.fft.ifftshift(ap * tcc))
+        tcc[m - 1] = Es * Ef * np.exp(-1j * 2 * np.pi * W / lam)
+    return tcc, np.asarray(gfsf, dtype=np.float64), ap
+
+
+def probe_stack(n, dx, kv, alpha_mrad, ab=None, mtotal=1, focal_rms_nm=0.0, source_mrad=0.0,
+                tilt_mrad=0.0, tilt_phi_deg=0.0):
+    """Probe wavefunctions [M, n, n] complex128 and focal-series weights gfsf [M]:
+    probe_m = fftshift(ifft2(ifftshift(ap * tcc_m))) / sqrt(sum |ifft2(...)|^2)
+    (``ToTEM_submit.m:788-806``), tcc_m and ap from :func:`tcc_stack`."""
+    tcc, gfsf, ap = tcc_stack(n, dx, kv, alpha_mrad, ab, mtotal, focal_rms_nm, source_mrad, tilt_mrad,
+                              tilt_phi_deg)
+    probes = []
+    for m in range(len(gfsf)):
+        w = np.fft.ifft2(np.fft.ifftshift(ap * tcc[m]))
         norm = np.sqrt(np.sum(np.abs(w) ** 2))
         probes.append(np.fft.fftshift(w) / norm)
-    return np.stack(probes), np.asarray(gfsf, dtype=np.float64)
+    return np.stack(probes), gfsf
 
 
 def unit_complex64(h):
