@@ -267,6 +267,39 @@ def probe_stack(n, dx, kv, alpha_mrad, ab=None, mtotal=1, focal_rms_nm=0.0, sour
     return np.stack(probes), gfsf
 
 
+def dpc_masks(n, dx, kv, rings):
+    """Segmented (4-quadrant) detector masks, centred, [4 * len(rings), n, n] (``makeAPER``,
+    ``ToTEM_submit.m:1139-1183``).  rings: (inner mrad, outer mrad, rotation degrees) per ring;
+    quadrants q = 1..4 of ring i are masks 4i..4i+3, each the ring times the angular sector
+    [phi0 + 90(q-1), phi0 + 90q) of atan2(ky, kx), phi0 derived from the rotation as the reference
+    does (rotations above 180 reduced by 180, then mapped to [-180, -90))."""
+    lam_A = wavelength_A(kv)
+    f = centred_freq(n, dx)
+    px, py = np.meshgrid(f, f)
+    p2 = px * px + py * py
+    theta = np.arctan2(py, px)
+    out = []
+    for k1, k2, rot in rings:
+        r1, r2 = k1 * 1e-3 / lam_A, k2 * 1e-3 / lam_A
+        ring = ((p2 >= r1 * r1) & (p2 <= r2 * r2)).astype(np.float64)
+        a = rot - 180 if rot > 180 else rot
+        if -180 <= a < -90:
+            d = a
+        elif -90 <= a < 0:
+            d = a - 90
+        elif 0 <= a < 90:
+            d = a - 180
+        elif 90 <= a < 180:
+            d = a - 270
+        else:
+            raise ValueError("detector rotation out of range: %r" % (rot,))
+        q = [((theta >= (d + 90 * j) * np.pi / 180) & (theta < (d + 90 * (j + 1)) * np.pi / 180)).astype(np.float64)
+             for j in range(3)]
+        q.append(1 - q[0] - q[1] - q[2])
+        out.extend(ring * m for m in q)
+    return np.stack(out)
+
+
 def unit_complex64(h):
     """complex128 -> complex64 keeping unit-modulus entries as close to |h| = 1 as FP32 allows.
 
