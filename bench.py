@@ -151,7 +151,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="capture the slice loop in a hipGraph")
     ap.add_argument("--scan", type=int, default=64, help="scan side of the 1024^2 workload (64: 64x64 probes)")
     ap.add_argument("--batch", type=int, default=0,
-                    help="probes in flight per launch (0: the engine's cache-fit batch)")
+                    help="probes in flight per launch (0: measured by the engine's autotuner)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the cache-fit rule instead of measuring")
     ap.add_argument("--plain-probes", type=int, default=16)
     ap.add_argument("--cpu-probes", type=int, default=16)
     ap.add_argument("--cpu-cores", type=int, default=16)
@@ -199,6 +200,9 @@ def main():
     job = engine.stem_job(N, NP, S, t.data_ptr(), probes.data_ptr(), M, prop.data_ptr(), aper.data_ptr(), A, D,
                           cfg.width_red, cfg.step, cfg.begin_row - 1, cfg.begin_col - 1, det.data_ptr(),
                           aper2_ptr=aper2.data_ptr(), wrap=True)
+    tuned = None
+    if args.batch <= 0 and not args.no_autotune:
+        tuned = engine.autotune(N)
     batch = min(M * D, args.batch if args.batch > 0 else engine.preferred_batch(N))
     ws_bytes = engine.stem_workspace_bytes(job, batch)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
@@ -297,6 +301,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded specimen and phonon snapshots)",
         "config": {"workload": name, "grid": N, "slices": S, "probes_per_step": M * D, "probes_per_launch": batch,
+                   "batch_autotuned": tuned is not None,
                    "phonon_configs_timed": args.steps, "paraflag": cfg.paraflag, "graph": bool(args.graph)},
         "plain": {"value": plain_value, "probes": pp,
                   "speedup_batched_vs_plain": value / plain_value if plain_value else None,

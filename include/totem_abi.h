@@ -143,6 +143,12 @@ typedef struct totem_stem_job {
  * memory-side cache (the batch myProjectfun uses; no counterpart in the
  * reference, which fills free memory, cuda_source_lobato.cu:1249-1260). */
 int totem_stem_preferred_batch(int n);
+/* Measures probes per launch for n x n grids on the current device (a synthetic job
+ * through `slices` <= 64 slices at batch sizes around the cache-fit rule) and makes
+ * the fastest the device's preferred batch for n from then on; returns it and its
+ * probe.slice/s (may be NULL).  Allocates and synchronises; no counterpart in the
+ * reference. */
+int totem_stem_autotune(int n, int slices, int *best_batch, double *probe_slices_per_s);
 /* Device bytes of workspace totem_stem_run needs for `batch` probes in flight. */
 size_t totem_stem_workspace_bytes(const totem_stem_job *job, int batch);
 /* Runs all M*D probes of the job, `batch` at a time (batch = 1 is the plain

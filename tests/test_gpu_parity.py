@@ -329,3 +329,12 @@ def test_full_size_vacuum_closed_form(engine):
         assert rel.max() < 1e-4
     print("  full-band intensity / N^2 - 1 = %.3e" % (out[2, 0] / n ** 2 - 1))
     assert out[2, 0] / n ** 2 == pytest.approx(1.0, abs=1e-5)
+
+
+def test_autotune_adopts_the_measured_batch(engine):
+    print("case autotune 256^2", flush=True)
+    rule = engine.preferred_batch(256)
+    b, rate = engine.autotune(256, 4)
+    print("  cache-fit rule %d, measured best %d (%.0f probe.slice/s)" % (rule, b, rate))
+    assert rule // 2 <= b <= 2 * rule and rate > 0
+    assert engine.preferred_batch(256) == b

@@ -469,7 +469,21 @@ void window_origin(const totem_stem_job& j, int d, int& r0, int& c0) {
 // the transmission slices stream from HBM.  Measured optimum 192-224 MiB of waves
 // for 256^2 to 2048^2 grids (1.1-1.2x the throughput of HBM-resident batches).
 constexpr size_t kCacheWaveBytes = 208ull << 20;
-int preferred_batch(int n) { return (int)std::max<size_t>(1, kCacheWaveBytes / ((size_t)n * n * sizeof(float2))); }
+static int cache_fit_batch(int n) { return (int)std::max<size_t>(1, kCacheWaveBytes / ((size_t)n * n * sizeof(float2))); }
+
+// Batches measured by totem_stem_autotune, per (device, n); they replace the rule.
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, int> g_tuned;
+
+int preferred_batch(int n) {
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find({dev, n});
+    if (it != g_tuned.end()) return it->second;
+  }
+  return cache_fit_batch(n);
+}
 
 // Per-kernel durations of the last timed run (TOTEM_RUN_TIMING), from HIP events
 // recorded on the launch stream around every rows/columns pass.
@@ -596,6 +610,77 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
     HIP_OK(hipGraphExecDestroy(gexec));
   }
   if (own.p) HIP_OK(hipStreamSynchronize(st));
+}
+
+// Probes per launch by measurement: a synthetic job (unit transmission and probe,
+// one detector) of 2 x b probes through `slices` slices for b around the cache-fit
+// rule; the fastest per probe.slice is kept for (device, n).
+static int autotune_impl(int n, int slices, double* best_rate) {
+  require(size_supported(n), "grid " + std::to_string(n) + " has no compiled FFT plan", TOTEM_ERR_UNSUPPORTED);
+  require(slices > 0 && slices <= 64, "autotune slices must be in 1..64");
+  hipStream_t st = lib_stream();
+  const int base = cache_fit_batch(n);
+  std::vector<int> cand;
+  for (double f : {0.5, 0.75, 1.0, 1.25, 1.5, 2.0}) {
+    const int b = std::max(1, (int)std::lround(base * f));
+    if (cand.empty() || cand.back() != b) cand.push_back(b);
+  }
+  const size_t nn = (size_t)n * n;
+  const int bmax = cand.back();
+  DevBuf t(slices * nn * sizeof(float2)), probe(nn * sizeof(float2)), prop(nn * sizeof(float2));
+  DevBuf aper(nn * sizeof(double)), det((size_t)2 * bmax * sizeof(double));
+  fill(st, t.as<float2>(), (long)(slices * nn), 1.f, 0.f);
+  fill(st, probe.as<float2>(), (long)nn, 1.f, 0.f);
+  fill(st, prop.as<float2>(), (long)nn, 1.f, 0.f);
+  fill(st, aper.as<double>(), (long)nn, 0.0);
+  totem_stem_job j;
+  std::memset(&j, 0, sizeof(j));
+  j.n = n;
+  j.np = n;
+  j.slices = slices;
+  j.t = t.p;
+  j.probes = probe.p;
+  j.n_probes = 1;
+  j.prop = prop.p;
+  j.aper = aper.as<double>();
+  j.n_aper = 1;
+  j.width_red = 1;
+  j.wrap = 1;
+  j.det = det.as<double>();
+  DevBuf ws;
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  int best = base;
+  double best_r = 0.0;
+  for (int b : cand) {
+    j.depth = 2 * b;
+    const size_t need = stem_ws_bytes(n, slices, 1, 0, b, 2 * b);
+    if (ws.bytes < need) ws.alloc(need);
+    run_stem_impl(j, b, 0, ws.p, ws.bytes, st);  // warm-up
+    HIP_OK(hipEventRecord(e0, st));
+    run_stem_impl(j, b, 0, ws.p, ws.bytes, st);
+    HIP_OK(hipEventRecord(e1, st));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    const double rate = 2.0 * b * slices / (ms * 1e-3);
+    if (rate > best_r) {
+      best_r = rate;
+      best = b;
+    }
+  }
+  HIP_OK(hipEventDestroy(e0));
+  HIP_OK(hipEventDestroy(e1));
+  HIP_OK(hipStreamSynchronize(st));
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[{dev, n}] = best;
+  }
+  if (best_rate) *best_rate = best_r;
+  return best;
 }
 
 // --------------------------------------------------------------------- myProjectfun
@@ -833,6 +918,13 @@ int totem_weighted_pattern(int n, int m, const void* fields, const double* weigh
 }
 
 int totem_stem_preferred_batch(int n) { return n > 0 ? totem::preferred_batch(n) : 0; }
+
+int totem_stem_autotune(int n, int slices, int* best_batch, double* probe_slices_per_s) {
+  return guarded([&] {
+    require(best_batch != nullptr, "best_batch is required");
+    *best_batch = totem::autotune_impl(n, slices, probe_slices_per_s);
+  });
+}
 
 size_t totem_stem_workspace_bytes(const totem_stem_job* job, int batch) {
   if (!job || batch <= 0 || !size_supported(job->n)) return 0;

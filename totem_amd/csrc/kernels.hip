@@ -597,6 +597,15 @@ k_accumulate(const float2* __restrict__ ex, const float2* __restrict__ ey, const
   }
 }
 
+__global__ void k_fill_c64(float2* __restrict__ p, long n, float re, float im) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_float2(re, im);
+}
+__global__ void k_fill_f64(double* __restrict__ p, long n, double v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // --------------------------------------------------------------------- launchers
 
 template <int N> void launch_pass_rows(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
@@ -689,6 +698,13 @@ void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, co
   if (total <= 0) return;
   hipLaunchKernelGGL(k_reduce_det, dim3((total + 255) / 256), dim3(256), 0, st, items, list, n, jobs, groups, lmax1,
                      amax);
+}
+
+void fill(hipStream_t st, float2* p, long n, float re, float im) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_c64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, re, im);
+}
+void fill(hipStream_t st, double* p, long n, double v) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 
 void fft2d(int n, hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi, float sigma) {

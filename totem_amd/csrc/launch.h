@@ -30,6 +30,9 @@ void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, co
 // In-place batched 2-D FFT of `batch` n x n complex64 matrices; the column pass
 // writes to dst (may equal data).  epi: 0 plain, 1 transmission exp(i sigma V), 2 scale 1/n^2.
 void fft2d(int n, hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi, float sigma);
+// p[0..n) = value (small fill kernels; graph-capture safe, unlike memset nodes)
+void fill(hipStream_t st, float2* p, long n, float re, float im);
+void fill(hipStream_t st, double* p, long n, double v);
 void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_terms, const double* kx1,
                   const double* ky1, int np, float2* ex, float2* ey);
 void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,

@@ -35,7 +35,7 @@ EXPORTS = (
     "totem_project_lobato", "totem_project_peng", "totem_potential_lobato", "totem_potential_peng",
     "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
     "totem_stem_workspace_bytes", "totem_stem_run", "totem_kernel_times", "totem_fft2d",
-    "totem_stem_preferred_batch", "totem_hrtem_images", "totem_tcc_image", "totem_weighted_pattern",
+    "totem_stem_preferred_batch", "totem_stem_autotune", "totem_hrtem_images", "totem_tcc_image", "totem_weighted_pattern",
     "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
     "totem_server_destroy",
 )
@@ -132,6 +132,8 @@ def lib():
         L.totem_tcc_image.restype = _I
         L.totem_weighted_pattern.argtypes = [_I, _I, _VP, _DP, _I, _DP, _VP]
         L.totem_weighted_pattern.restype = _I
+        L.totem_stem_autotune.argtypes = [_I, _I, ctypes.POINTER(_I), _DP]
+        L.totem_stem_autotune.restype = _I
         L.totem_stem_preferred_batch.argtypes = [_I]
         L.totem_stem_preferred_batch.restype = _I
         L.totem_server_create.argtypes = [ctypes.c_size_t, _I, _I, ctypes.POINTER(_VP)]
@@ -340,6 +342,14 @@ def weighted_pattern(fields_ptr, n, weights, shift=True):
     out = np.zeros((n, n))
     check(lib().totem_weighted_pattern(int(n), int(w.size), _VP(fields_ptr), _dp(w), int(bool(shift)), _dp(out), None))
     return out
+
+
+def autotune(n, slices=8):
+    """Measure and adopt the probes-per-launch for n x n grids; returns (batch, probe.slice/s)."""
+    b = _I()
+    r = ctypes.c_double()
+    check(lib().totem_stem_autotune(int(n), int(slices), ctypes.byref(b), ctypes.byref(r)))
+    return b.value, r.value
 
 
 def stem_workspace_bytes(job, batch):
