@@ -338,3 +338,17 @@ def test_autotune_adopts_the_measured_batch(engine):
     print("  cache-fit rule %d, measured best %d (%.0f probe.slice/s)" % (rule, b, rate))
     assert rule // 2 <= b <= 2 * rule and rate > 0
     assert engine.preferred_batch(256) == b
+
+
+def test_small_lobato_workload_full_scan_matches_oracle(engine):
+    """The small Lobato workload (SrTiO3 5x5x10 cells, 1250 atoms, 256^2 over 19.525 A, 20 slices,
+    200 kV, 20 mrad, ADF 60-200 mrad, 16 x 16 positions at 3 px) with 128^2 probe windows inside the
+    potential (no window wraps), through myProjectfun, against the FP64 oracle per pixel."""
+    print("case small Lobato workload 16x16 scan", flush=True)
+    cfg = stem.perovskite_c1(n=128, np_side=256, scan=16, step=3)
+    cfg.begin_row, cfg.begin_col = 41, 37
+    inp = stem.prepare(cfg)
+    myresul, _, _, _ = stem.run(inp)
+    ref, _ = _oracle_stem(inp)
+    assert myresul.shape == (256, 1)
+    _check_images(myresul, ref, "ADF 60-200 mrad image (256 positions)")

@@ -100,14 +100,17 @@ def test_multiceng_weights_sum_to_one():
     st = np.arange(6) * a / 2 - 0.3
     ele_n, _, sn, _ = specimen.slice_atoms(all_n, all_n_i, a / 2, st, 1, "p", 0.0)
     assert len(ele_n) > len(all_n)
-    per_atom = {}
+    # the weighted a1 of every atom column (same x, y) summed over slices equals its atoms' a1 sum
+    got, ref = {}, {}
     for r in ele_n:
-        key = (round(r[0], 9), round(r[1], 9), round(r[3] / tables.peng_table()[int(0) + 7, 0], 3))
-        per_atom.setdefault((round(r[0], 9), round(r[1], 9)), []).append(r[3])
-    # sum of a1 weights over slices equals the table value for every column of atoms
-    tot = sum(ele_n[:, 3])
-    ref = sum(r[6] for r in all_n)
-    assert tot == pytest.approx(ref, rel=1e-9)
+        k = (round(r[0], 9), round(r[1], 9))
+        got[k] = got.get(k, 0.0) + r[3]
+    for r in all_n:
+        k = (round(r[1], 9), round(r[2], 9))
+        ref[k] = ref.get(k, 0.0) + r[6]
+    assert got.keys() == ref.keys()
+    for k in ref:
+        assert got[k] == pytest.approx(ref[k], rel=1e-9)
 
 
 def test_corr_slicing_classes():
