@@ -161,6 +161,39 @@ int totem_stem_run(const totem_stem_job *job, int batch, int flags, void *worksp
  * over TOTEM_RUN_TIMING runs on the calling thread; reset != 0 clears them after reading. */
 void totem_kernel_times(int reset, double *rows_ms, long *rows_n, double *cols_ms, long *cols_n);
 
+/* ----------------------------------------------- specimen on the device
+ * Frozen-phonon displacement and slicing on the GPU (the host steps before
+ * myProjectfun in the reference: randshiftpos.m:1-44 and
+ * CalAllEquilent20210223_lobato_peng.m:23-57 with multiceng = 0,
+ * CalAllEquilent20210305_peng_corr.m:25-70 without half-splitting).  `pot`
+ * holds every atom's coefficient rows as one slice (slices = 1; grids, sigma,
+ * parameter, correction tables as for totem_build_transmission); term_atom maps
+ * each of its rows, in the order proj_coff_mat, proj_coff_mat_i, ele_n_corr,
+ * ele_n_i_corr, to its atom. */
+typedef struct totem_specimen_desc {
+  totem_potential_desc pot;
+  int n_atoms;
+  const double *xyz;      /* [n_atoms][3] positions (Angstrom) */
+  const double *rms;      /* [n_atoms] displacement rms per axis (Angstrom), or NULL (static specimen) */
+  const int *term_atom;   /* [n_terms] */
+  int n_terms;
+  int slices;             /* S */
+  double slice_origin;    /* st[0]; slice i is st[i] <= z < st[i] + slice_thickness, st[i] = st[0] + i dz */
+  double slice_thickness;
+  const int *mid;         /* [n_mid] 1-based slices whose lower face st[m-1] + dz atoms must not cross */
+  int n_mid;
+} totem_specimen_desc;
+typedef struct totem_specimen totem_specimen;
+int totem_specimen_create(const totem_specimen_desc *desc, totem_specimen **out);
+/* Displaces the atoms for (seed, config) -- Gaussian per axis, redrawn (up to 64 times, then
+ * left in place) while an atom leaves [st[0], st[S-1] + dz) or crosses a mid-slice face --
+ * slices them and builds t = exp(i sigma/1000 V) into t_dev (complex64 [S][np][np]). */
+int totem_specimen_transmission(totem_specimen *spec, unsigned long long seed, int config, void *t_dev,
+                                void *stream);
+/* The displaced positions of (seed, config), host [n_atoms][3]. */
+int totem_specimen_positions(totem_specimen *spec, unsigned long long seed, int config, double *xyz);
+int totem_specimen_destroy(totem_specimen *spec);
+
 /* ------------------------------------------------------------- served engine
  * Continuous batching of many device STEM jobs (no counterpart in the reference,
  * which runs one myProjectfun call at a time): jobs of any supported grid side are

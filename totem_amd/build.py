@@ -42,7 +42,7 @@ def _run(cmd):
 def build(verbose=True):
     os.makedirs(LIB, exist_ok=True)
     objs = []
-    for src in ("kernels.hip", "engine.hip", "server.hip", "hrtem.hip"):
+    for src in ("kernels.hip", "engine.hip", "server.hip", "hrtem.hip", "specimen.hip"):
         obj = os.path.join(LIB, src.replace(".hip", ".o"))
         _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(SRC, src), "-o", obj])
         objs.append(obj)

@@ -274,23 +274,82 @@ static PotentialPlan plan_potential(const totem_potential_desc& d) {
         P.slices[s].push_back({r[0], r[1], r[3], class_of(std::move(key))});
       }
   }
+  // Class ids in key order: any subset of the same classes keeps the same
+  // relative order, so the per-slice segment order (and with it the summation
+  // order) does not depend on which other atoms the specimen holds.
+  std::vector<int> remap(P.classes.size());
+  int next = 0;
+  for (const auto& kv : cmap) remap[kv.second] = next++;
+  std::vector<PotClass> sorted(P.classes.size());
+  for (size_t c = 0; c < P.classes.size(); ++c) sorted[remap[c]] = P.classes[c];
+  P.classes.swap(sorted);
+  for (auto& sl : P.slices)
+    for (auto& t : sl) t.cls = remap[t.cls];
   return P;
 }
 
-static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev, double* v_re, double* v_im,
-                                    hipStream_t st) {
-  PotentialPlan P = plan_potential(d);
-  const int np = P.np, S = P.S;
-  const size_t np2 = (size_t)np * np;
-  // flatten terms slice by slice, grouped by class within a slice
-  std::vector<double> tx, ty, wt;
-  std::vector<PotSeg> segs;
-  std::vector<long> term_off(S + 1, 0), seg_off(S + 1, 0);
+// Device-side inputs of one potential build: terms flattened slice by slice and
+// grouped by form-factor class within a slice (segments), class table, grids.
+struct PotDev {
+  int np = 0, S = 0;
+  const double *kx1 = nullptr, *ky1 = nullptr, *s2 = nullptr, *aper = nullptr, *corr = nullptr;
+  const PotClass* classes = nullptr;
+  const double *tx = nullptr, *ty = nullptr, *wt = nullptr;
+  const PotSeg* segs = nullptr;
+  std::vector<long> term_off, seg_off;  // host, [S + 1]
   size_t max_terms = 0;
+  double sigma = 0.0, parameter = 0.0;
+};
+
+// Per slice: phase tables, reciprocal-space accumulation, unnormalised inverse FFT
+// with exp(i sigma/1000 V) fused into its column pass (or V copied out when asked).
+void potential_run(const PotDev& p, float2* t_dev, double* v_re, double* v_im, hipStream_t st, DevBuf& dex,
+                   DevBuf& dey) {
+  const int np = p.np, S = p.S;
+  const size_t np2 = (size_t)np * np;
+  reserve(dex, std::max<size_t>(p.max_terms, 1) * np * sizeof(float2));
+  reserve(dey, std::max<size_t>(p.max_terms, 1) * np * sizeof(float2));
+  tw_table(np);
+  const float sig = (float)(p.sigma / 1000.0);
+  const bool want_v = v_re && v_im;
+  std::vector<float2> hv;
+  if (want_v) hv.resize(np2);
+  for (int s = 0; s < S; ++s) {
+    const int nt = (int)(p.term_off[s + 1] - p.term_off[s]);
+    float2* ts = t_dev + (size_t)s * np2;
+    phase_tables(st, p.tx + p.term_off[s], p.ty + p.term_off[s], nt, p.kx1, p.ky1, np, dex.as<float2>(),
+                 dey.as<float2>());
+    accumulate(st, dex.as<float2>(), dey.as<float2>(), p.wt + p.term_off[s], p.segs + p.seg_off[s],
+               (int)(p.seg_off[s + 1] - p.seg_off[s]), p.classes, p.s2, p.aper, p.corr, p.parameter, np, ts);
+    if (want_v) {
+      fft2d(np, st, ts, ts, 1, +1, 0, 0.f);
+      HIP_OK(hipMemcpyAsync(hv.data(), ts, np2 * sizeof(float2), hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      for (size_t i = 0; i < np2; ++i) {
+        v_re[(size_t)s * np2 + i] = hv[i].x;
+        v_im[(size_t)s * np2 + i] = hv[i].y;
+        const float amp = std::exp(-sig * hv[i].y);
+        hv[i] = make_float2(amp * std::cos(sig * hv[i].x), amp * std::sin(sig * hv[i].x));
+      }
+      HIP_OK(hipMemcpyAsync(ts, hv.data(), np2 * sizeof(float2), hipMemcpyHostToDevice, st));
+    } else {
+      fft2d(np, st, ts, ts, 1, +1, 1, sig);
+    }
+    HIP_OK(hipGetLastError());
+  }
+}
+
+// Flattens a plan's terms slice by slice, stably grouped by class within a slice.
+static void flatten_plan(PotentialPlan& P, std::vector<double>& tx, std::vector<double>& ty, std::vector<double>& wt,
+                         std::vector<PotSeg>& segs, std::vector<long>& term_off, std::vector<long>& seg_off,
+                         size_t& max_terms) {
+  const int S = P.S;
+  term_off.assign(S + 1, 0);
+  seg_off.assign(S + 1, 0);
+  max_terms = 0;
   for (int s = 0; s < S; ++s) {
     auto& v = P.slices[s];
     std::stable_sort(v.begin(), v.end(), [](const Term& a, const Term& b) { return a.cls < b.cls; });
-    const long base = (long)tx.size();
     for (size_t i = 0; i < v.size(); ++i) {
       tx.push_back(v[i].x);
       ty.push_back(v[i].y);
@@ -298,11 +357,20 @@ static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev
       if (i == 0 || v[i].cls != v[i - 1].cls) segs.push_back({v[i].cls, (int)i, (int)i + 1});
       else segs.back().end = (int)i + 1;
     }
-    (void)base;
     term_off[s + 1] = (long)tx.size();
     seg_off[s + 1] = (long)segs.size();
     max_terms = std::max(max_terms, v.size());
   }
+}
+
+static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev, double* v_re, double* v_im,
+                                    hipStream_t st) {
+  PotentialPlan P = plan_potential(d);
+  const size_t np2 = (size_t)P.np * P.np;
+  std::vector<double> tx, ty, wt;
+  std::vector<PotSeg> segs;
+  PotDev pd;
+  flatten_plan(P, tx, ty, wt, segs, pd.term_off, pd.seg_off, pd.max_terms);
   // per-device builder scratch, one build at a time per device
   struct Scratch {
     std::mutex mu;
@@ -320,55 +388,205 @@ static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev
     sc = p.get();
   }
   std::lock_guard<std::mutex> build_lock(sc->mu);
-  DevBuf &dtx = sc->dtx, &dty = sc->dty, &dwt = sc->dwt, &dsegs = sc->dsegs, &dcls = sc->dcls, &dkx = sc->dkx,
-         &dky = sc->dky, &ds2 = sc->ds2, &dap = sc->dap, &dcorr = sc->dcorr;
-  upload(dtx, tx, st);
-  upload(dty, ty, st);
-  upload(dwt, wt, st);
-  upload(dsegs, segs, st);
-  upload(dcls, P.classes, st);
-  upload(dkx, P.kx1, st);
-  upload(dky, P.ky1, st);
-  reserve(ds2, np2 * sizeof(double));
-  HIP_OK(hipMemcpyAsync(ds2.p, d.s_2, np2 * sizeof(double), hipMemcpyHostToDevice, st));
-  reserve(dap, np2 * sizeof(double));
-  HIP_OK(hipMemcpyAsync(dap.p, d.aper_true, np2 * sizeof(double), hipMemcpyHostToDevice, st));
+  upload(sc->dtx, tx, st);
+  upload(sc->dty, ty, st);
+  upload(sc->dwt, wt, st);
+  upload(sc->dsegs, segs, st);
+  upload(sc->dcls, P.classes, st);
+  upload(sc->dkx, P.kx1, st);
+  upload(sc->dky, P.ky1, st);
+  reserve(sc->ds2, np2 * sizeof(double));
+  HIP_OK(hipMemcpyAsync(sc->ds2.p, d.s_2, np2 * sizeof(double), hipMemcpyHostToDevice, st));
+  reserve(sc->dap, np2 * sizeof(double));
+  HIP_OK(hipMemcpyAsync(sc->dap.p, d.aper_true, np2 * sizeof(double), hipMemcpyHostToDevice, st));
   if (d.cim_len > 0) {
-    reserve(dcorr, np2 * d.cim_len * sizeof(double));
-    HIP_OK(hipMemcpyAsync(dcorr.p, d.corr_info_matrix, np2 * d.cim_len * sizeof(double), hipMemcpyHostToDevice, st));
+    reserve(sc->dcorr, np2 * d.cim_len * sizeof(double));
+    HIP_OK(hipMemcpyAsync(sc->dcorr.p, d.corr_info_matrix, np2 * d.cim_len * sizeof(double), hipMemcpyHostToDevice,
+                          st));
   }
-  DevBuf &dex = sc->dex, &dey = sc->dey;
-  reserve(dex, std::max<size_t>(max_terms, 1) * np * sizeof(float2));
-  reserve(dey, std::max<size_t>(max_terms, 1) * np * sizeof(float2));
-  tw_table(np);
-  const float sig = (float)(d.sigma / 1000.0);
-  const bool want_v = v_re && v_im;
-  std::vector<float2> hv;
-  if (want_v) hv.resize(np2);
+  pd.np = P.np;
+  pd.S = P.S;
+  pd.kx1 = sc->dkx.as<double>();
+  pd.ky1 = sc->dky.as<double>();
+  pd.s2 = sc->ds2.as<double>();
+  pd.aper = sc->dap.as<double>();
+  pd.corr = sc->dcorr.as<double>();
+  pd.classes = sc->dcls.as<PotClass>();
+  pd.tx = sc->dtx.as<double>();
+  pd.ty = sc->dty.as<double>();
+  pd.wt = sc->dwt.as<double>();
+  pd.segs = sc->dsegs.as<PotSeg>();
+  pd.sigma = d.sigma;
+  pd.parameter = d.parameter;
+  potential_run(pd, t_dev, v_re, v_im, st, sc->dex, sc->dey);
+  HIP_OK(hipStreamSynchronize(st));
+}
+
+// ------------------------------------------------------- specimen on the device
+
+}  // namespace totem
+
+// Resident specimen: static atoms and per-term data on the device; per phonon
+// configuration the atoms are displaced, sliced and the transmission built
+// without leaving the GPU except for the per-(slice, class) counts.
+struct totem_specimen {
+  std::mutex mu;
+  int dev = 0;
+  int np = 0, S = 0, C = 0, n_atoms = 0, n_terms = 0, n_faces = 0, bits = 1;
+  double origin = 0, dz = 0, lo = 0, hi = 0, sigma = 0, parameter = 0;
+  bool has_rms = false;
+  size_t sort_bytes = 0;
+  totem::DevBuf xyz, rms, pos, faces, term_atom, term_cls, term_w, keys, keys2, vals, vals2, counts, sort_tmp;
+  totem::DevBuf tx, ty, wt, segs, classes, kx, ky, s2, aper, corr, ex, ey;
+  std::vector<int> h_counts;
+  std::vector<totem::PotSeg> h_segs;
+};
+
+namespace totem {
+
+static totem_specimen* specimen_create_impl(const totem_specimen_desc& d) {
+  require(d.pot.slices == 1, "specimen: pot must hold every atom's rows as one slice (slices = 1)");
+  require(d.slices > 0, "specimen: slices must be positive");
+  require(d.slice_thickness > 0 && std::isfinite(d.slice_thickness) && std::isfinite(d.slice_origin),
+          "specimen: slice_thickness must be positive and finite");
+  require(d.n_atoms >= 0 && (d.n_atoms == 0 || d.xyz), "specimen: xyz missing");
+  require(d.n_terms >= 0 && (d.n_terms == 0 || d.term_atom), "specimen: term_atom missing");
+  require(d.n_mid >= 0 && (d.n_mid == 0 || d.mid), "specimen: mid missing");
+  PotentialPlan P = plan_potential(d.pot);
+  const auto& terms = P.slices[0];
+  require((long)terms.size() == (long)d.n_terms,
+          "specimen: n_terms = " + std::to_string(d.n_terms) + " but pot holds " + std::to_string(terms.size()) +
+              " rows");
+  const int np = P.np, S = d.slices, C = std::max<int>(1, (int)P.classes.size());
+  require((long)S * C < (1L << 30), "specimen: slices x form-factor classes exceeds 2^30");
+  std::vector<int> t_atom(d.term_atom, d.term_atom + d.n_terms), t_cls(d.n_terms);
+  std::vector<double> t_w(d.n_terms);
+  for (int i = 0; i < d.n_terms; ++i) {
+    const int a = t_atom[i];
+    require(a >= 0 && a < d.n_atoms, "specimen: term_atom out of range");
+    require(terms[i].x == d.xyz[3 * a] && terms[i].y == d.xyz[3 * a + 1],
+            "specimen: row " + std::to_string(i) + " does not sit at its atom's x, y");
+    t_cls[i] = terms[i].cls;
+    t_w[i] = terms[i].w;
+  }
+  std::vector<double> faces;
+  for (int m = 0; m < d.n_mid; ++m) {
+    require(d.mid[m] >= 1 && d.mid[m] <= S, "specimen: mid slice out of range");
+    faces.push_back(d.slice_origin + d.slice_thickness * (double)(d.mid[m] - 1) + d.slice_thickness);
+  }
+  std::vector<double> xyz(d.xyz, d.xyz + 3 * (size_t)d.n_atoms), rms;
+  if (d.rms) {
+    rms.assign(d.rms, d.rms + d.n_atoms);
+    for (double r : rms) require(r >= 0 && std::isfinite(r), "specimen: rms must be finite and >= 0");
+  }
+  std::unique_ptr<totem_specimen> sp(new totem_specimen());
+  HIP_OK(hipGetDevice(&sp->dev));
+  sp->np = np;
+  sp->S = S;
+  sp->C = C;
+  sp->n_atoms = d.n_atoms;
+  sp->n_terms = d.n_terms;
+  sp->n_faces = (int)faces.size();
+  sp->origin = d.slice_origin;
+  sp->dz = d.slice_thickness;
+  sp->lo = d.slice_origin;                                                         // st[0]
+  sp->hi = d.slice_origin + d.slice_thickness * (double)(S - 1) + d.slice_thickness;  // st[S-1] + dz
+  sp->sigma = d.pot.sigma;
+  sp->parameter = d.pot.parameter;
+  sp->has_rms = d.rms != nullptr;
+  while ((1L << sp->bits) <= (long)S * C) ++sp->bits;
+  hipStream_t st = lib_stream();
+  const size_t np2 = (size_t)np * np, n2 = 2 * (size_t)std::max(d.n_terms, 1);
+  upload(sp->xyz, xyz, st);
+  upload(sp->rms, rms, st);
+  upload(sp->faces, faces, st);
+  upload(sp->term_atom, t_atom, st);
+  upload(sp->term_cls, t_cls, st);
+  upload(sp->term_w, t_w, st);
+  upload(sp->classes, P.classes, st);
+  upload(sp->kx, P.kx1, st);
+  upload(sp->ky, P.ky1, st);
+  reserve(sp->pos, std::max<size_t>(3 * (size_t)d.n_atoms, 1) * sizeof(double));
+  reserve(sp->s2, np2 * sizeof(double));
+  HIP_OK(hipMemcpyAsync(sp->s2.p, d.pot.s_2, np2 * sizeof(double), hipMemcpyHostToDevice, st));
+  reserve(sp->aper, np2 * sizeof(double));
+  HIP_OK(hipMemcpyAsync(sp->aper.p, d.pot.aper_true, np2 * sizeof(double), hipMemcpyHostToDevice, st));
+  if (d.pot.cim_len > 0) {
+    reserve(sp->corr, np2 * d.pot.cim_len * sizeof(double));
+    HIP_OK(hipMemcpyAsync(sp->corr.p, d.pot.corr_info_matrix, np2 * d.pot.cim_len * sizeof(double),
+                          hipMemcpyHostToDevice, st));
+  }
+  for (DevBuf* b : {&sp->keys, &sp->keys2, &sp->vals, &sp->vals2}) reserve(*b, n2 * sizeof(unsigned));
+  reserve(sp->counts, (size_t)S * C * sizeof(int));
+  for (DevBuf* b : {&sp->tx, &sp->ty, &sp->wt}) reserve(*b, n2 * sizeof(double));
+  reserve(sp->segs, std::max<size_t>((size_t)S * C, 1) * sizeof(PotSeg));
+  sp->sort_bytes = sort_pairs_temp_bytes((int)n2, sp->bits);
+  reserve(sp->sort_tmp, sp->sort_bytes);
+  sp->h_counts.resize((size_t)S * C);
+  HIP_OK(hipStreamSynchronize(st));
+  return sp.release();
+}
+
+static void specimen_positions_impl(totem_specimen& sp, uint64_t seed, int config, hipStream_t st) {
+  displace(st, sp.n_atoms, sp.xyz.as<double>(), sp.has_rms ? sp.rms.as<double>() : nullptr, seed, config, sp.lo,
+           sp.hi, sp.faces.as<double>(), sp.n_faces, sp.pos.as<double>());
+  HIP_OK(hipGetLastError());
+}
+
+static void specimen_transmission_impl(totem_specimen& sp, uint64_t seed, int config, float2* t_dev,
+                                       hipStream_t st) {
+  const int S = sp.S, C = sp.C, n2 = 2 * sp.n_terms;
+  specimen_positions_impl(sp, seed, config, st);
+  term_keys(st, sp.n_terms, sp.term_atom.as<int>(), sp.term_cls.as<int>(), sp.pos.as<double>(), S, C, sp.origin,
+            sp.dz, sp.keys.as<unsigned>(), sp.vals.as<unsigned>(), sp.counts.as<int>());
+  HIP_OK(hipGetLastError());
+  if (n2 > 0)
+    HIP_OK(sort_pairs(st, sp.sort_tmp.p, sp.sort_bytes, sp.keys.as<unsigned>(), sp.keys2.as<unsigned>(),
+                      sp.vals.as<unsigned>(), sp.vals2.as<unsigned>(), n2, sp.bits));
+  HIP_OK(hipMemcpyAsync(sp.h_counts.data(), sp.counts.p, sp.h_counts.size() * sizeof(int), hipMemcpyDeviceToHost,
+                        st));
+  HIP_OK(hipStreamSynchronize(st));
+  // Per slice: terms [term_off[s], term_off[s+1]) of the sorted order, one
+  // segment per class present, classes ascending -- the host build's layout.
+  PotDev pd;
+  pd.term_off.assign(S + 1, 0);
+  pd.seg_off.assign(S + 1, 0);
+  sp.h_segs.clear();
+  long total = 0;
   for (int s = 0; s < S; ++s) {
-    const int nt = (int)(term_off[s + 1] - term_off[s]);
-    float2* ts = t_dev + (size_t)s * np2;
-    phase_tables(st, dtx.as<double>() + term_off[s], dty.as<double>() + term_off[s], nt, dkx.as<double>(),
-                 dky.as<double>(), np, dex.as<float2>(), dey.as<float2>());
-    accumulate(st, dex.as<float2>(), dey.as<float2>(), dwt.as<double>() + term_off[s],
-               dsegs.as<PotSeg>() + seg_off[s], (int)(seg_off[s + 1] - seg_off[s]), dcls.as<PotClass>(),
-               ds2.as<double>(), dap.as<double>(), dcorr.as<double>(), d.parameter, np, ts);
-    if (want_v) {
-      fft2d(np, st, ts, ts, 1, +1, 0, 0.f);
-      HIP_OK(hipMemcpyAsync(hv.data(), ts, np2 * sizeof(float2), hipMemcpyDeviceToHost, st));
-      HIP_OK(hipStreamSynchronize(st));
-      for (size_t i = 0; i < np2; ++i) {
-        v_re[(size_t)s * np2 + i] = hv[i].x;
-        v_im[(size_t)s * np2 + i] = hv[i].y;
-        const float amp = std::exp(-sig * hv[i].y);
-        hv[i] = make_float2(amp * std::cos(sig * hv[i].x), amp * std::sin(sig * hv[i].x));
-      }
-      HIP_OK(hipMemcpyAsync(ts, hv.data(), np2 * sizeof(float2), hipMemcpyHostToDevice, st));
-    } else {
-      fft2d(np, st, ts, ts, 1, +1, 1, sig);
+    int in_slice = 0;
+    for (int c = 0; c < C; ++c) {
+      const int k = sp.h_counts[(size_t)s * C + c];
+      if (k > 0) sp.h_segs.push_back({c, in_slice, in_slice + k});
+      in_slice += k;
     }
-    HIP_OK(hipGetLastError());
+    total += in_slice;
+    pd.term_off[s + 1] = total;
+    pd.seg_off[s + 1] = (long)sp.h_segs.size();
+    pd.max_terms = std::max<size_t>(pd.max_terms, (size_t)in_slice);
   }
+  require(total <= n2, "specimen: internal count mismatch");
+  if (!sp.h_segs.empty())
+    HIP_OK(hipMemcpyAsync(sp.segs.p, sp.h_segs.data(), sp.h_segs.size() * sizeof(PotSeg), hipMemcpyHostToDevice,
+                          st));
+  term_gather(st, (int)total, sp.vals2.as<unsigned>(), sp.term_atom.as<int>(), sp.term_w.as<double>(),
+              sp.pos.as<double>(), sp.tx.as<double>(), sp.ty.as<double>(), sp.wt.as<double>());
+  HIP_OK(hipGetLastError());
+  pd.np = sp.np;
+  pd.S = S;
+  pd.kx1 = sp.kx.as<double>();
+  pd.ky1 = sp.ky.as<double>();
+  pd.s2 = sp.s2.as<double>();
+  pd.aper = sp.aper.as<double>();
+  pd.corr = sp.corr.as<double>();
+  pd.classes = sp.classes.as<PotClass>();
+  pd.tx = sp.tx.as<double>();
+  pd.ty = sp.ty.as<double>();
+  pd.wt = sp.wt.as<double>();
+  pd.segs = sp.segs.as<PotSeg>();
+  pd.sigma = sp.sigma;
+  pd.parameter = sp.parameter;
+  potential_run(pd, t_dev, nullptr, nullptr, st, sp.ex, sp.ey);
   HIP_OK(hipStreamSynchronize(st));
 }
 
@@ -914,6 +1132,47 @@ int totem_weighted_pattern(int n, int m, const void* fields, const double* weigh
   return guarded([&] {
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : totem::lib_stream();
     totem::weighted_pattern_impl(n, m, fields, weights, shift, out, st);
+  });
+}
+
+int totem_specimen_create(const totem_specimen_desc* desc, totem_specimen** out) {
+  return guarded([&] {
+    require(desc && out, "null argument");
+    *out = nullptr;
+    *out = specimen_create_impl(*desc);
+  });
+}
+
+int totem_specimen_transmission(totem_specimen* spec, unsigned long long seed, int config, void* t_dev,
+                                void* stream) {
+  return guarded([&] {
+    require(spec && t_dev, "null argument");
+    std::lock_guard<std::mutex> lk(spec->mu);
+    HIP_OK(hipSetDevice(spec->dev));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : lib_stream();
+    specimen_transmission_impl(*spec, seed, config, static_cast<float2*>(t_dev), st);
+  });
+}
+
+int totem_specimen_positions(totem_specimen* spec, unsigned long long seed, int config, double* xyz) {
+  return guarded([&] {
+    require(spec && xyz, "null argument");
+    std::lock_guard<std::mutex> lk(spec->mu);
+    HIP_OK(hipSetDevice(spec->dev));
+    hipStream_t st = lib_stream();
+    specimen_positions_impl(*spec, seed, config, st);
+    if (spec->n_atoms > 0)
+      HIP_OK(hipMemcpyAsync(xyz, spec->pos.p, 3 * (size_t)spec->n_atoms * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+  });
+}
+
+int totem_specimen_destroy(totem_specimen* spec) {
+  return guarded([&] {
+    if (!spec) return;
+    HIP_OK(hipSetDevice(spec->dev));
+    HIP_OK(hipDeviceSynchronize());
+    delete spec;
   });
 }
 

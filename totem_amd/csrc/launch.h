@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include "kernels.h"
 
+#include <cstdint>
 #include <vector>
 
 namespace totem {
@@ -38,5 +39,17 @@ void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_term
 void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,
                 const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
                 double parameter, int np, float2* vshift);
+
+// Specimen slicing on the device (specimen.hip).  term_keys clears counts[S * C],
+// then writes keys/vals[2 n] (two slots per term) and counts terms per key.
+void displace(hipStream_t st, int n, const double* xyz, const double* rms, uint64_t seed, int config, double lo,
+              double hi, const double* faces, int n_faces, double* out);
+void term_keys(hipStream_t st, int n, const int* term_atom, const int* term_cls, const double* xyz, int S, int C,
+               double origin, double dz, unsigned* keys, unsigned* vals, int* counts);
+size_t sort_pairs_temp_bytes(int n, int bits);  // stable radix sort of (key, value) pairs on bits [0, bits)
+hipError_t sort_pairs(hipStream_t st, void* temp, size_t temp_bytes, const unsigned* keys_in, unsigned* keys_out,
+                      const unsigned* vals_in, unsigned* vals_out, int n, int bits);
+void term_gather(hipStream_t st, int n, const unsigned* vals, const int* term_atom, const double* term_w,
+                 const double* xyz, double* tx, double* ty, double* wt);
 
 }  // namespace totem

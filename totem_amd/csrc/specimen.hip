@@ -1,0 +1,175 @@
+// Specimen slicing and frozen-phonon displacement on the GPU.
+//
+// The reference displaces atoms on the host (randshiftpos.m:1-44: Gaussian
+// per axis, redrawn while an atom leaves the slab or crosses a mid-slice face)
+// and assigns them to slices on the host (CalAllEquilent20210223_lobato_peng.m:
+// 23-57 with multiceng = 0: slice i holds st[i] <= z < st[i] + dz).  Here both
+// run per phonon configuration on the device, on terms that carry their atom
+// and form-factor class: displaced positions -> (slice, class) keys -> stable
+// radix sort -> term arrays grouped by slice and class, the layout the
+// potential kernels read.  Random numbers: Philox4x32-10 (Salmon et al., SC'11),
+// counter = (atom, configuration, attempt, lane), key = seed; normals by
+// Box-Muller in double precision.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cstdint>
+
+#include "launch.h"
+
+namespace totem {
+
+namespace {
+
+__device__ __forceinline__ void philox_round(uint32_t c[4], const uint32_t k[2]) {
+  const uint32_t m0 = 0xD2511F53u, m1 = 0xCD9E8D57u;
+  const uint32_t hi0 = __umulhi(m0, c[0]), lo0 = m0 * c[0];
+  const uint32_t hi1 = __umulhi(m1, c[2]), lo1 = m1 * c[2];
+  const uint32_t o0 = hi1 ^ c[1] ^ k[0], o2 = hi0 ^ c[3] ^ k[1];
+  c[0] = o0;
+  c[1] = lo1;
+  c[2] = o2;
+  c[3] = lo0;
+}
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint64_t seed) {
+  uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    philox_round(c, k);
+    k[0] += 0x9E3779B9u;
+    k[1] += 0xBB67AE85u;
+  }
+}
+
+// uniform in (0, 1) from 53 random bits
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ void normals3(uint64_t seed, int atom, int config, int attempt, double n[3]) {
+  uint32_t c[4] = {(uint32_t)atom, (uint32_t)config, (uint32_t)attempt, 0u};
+  philox4x32_10(c, seed);
+  const double r = sqrt(-2.0 * log(u53(c[0], c[1]))), th = 2.0 * M_PI * u53(c[2], c[3]);
+  n[0] = r * cos(th);
+  n[1] = r * sin(th);
+  uint32_t d[4] = {(uint32_t)atom, (uint32_t)config, (uint32_t)attempt, 1u};
+  philox4x32_10(d, seed);
+  const double r2 = sqrt(-2.0 * log(u53(d[0], d[1]))), th2 = 2.0 * M_PI * u53(d[2], d[3]);
+  n[2] = r2 * cos(th2);
+}
+
+constexpr int kMaxAttempts = 64;
+
+__global__ void k_displace(int n, const double* __restrict__ xyz, const double* __restrict__ rms, uint64_t seed,
+                           int config, double lo, double hi, const double* __restrict__ faces, int n_faces,
+                           double* __restrict__ out) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const double x = xyz[3 * a], y = xyz[3 * a + 1], z = xyz[3 * a + 2];
+  double px = x, py = y, pz = z;
+  const double s = rms ? rms[a] : 0.0;
+  if (s > 0.0) {
+    for (int k = 0; k < kMaxAttempts; ++k) {
+      double g[3];
+      normals3(seed, a, config, k, g);
+      const double cx = x + s * g[0], cy = y + s * g[1], cz = z + s * g[2];
+      bool ok = cz >= lo && cz < hi;
+      for (int f = 0; f < n_faces && ok; ++f) ok = (z < faces[f]) == (cz < faces[f]);
+      if (ok) {
+        px = cx;
+        py = cy;
+        pz = cz;
+        break;
+      }
+    }
+  }
+  out[3 * a] = px;
+  out[3 * a + 1] = py;
+  out[3 * a + 2] = pz;
+}
+
+// Two key slots per term: slice i holds st[i] <= z < st[i] + dz with
+// st[i] = origin + dz * i, the host's test, so an atom within rounding of a
+// face can sit in two adjacent slices (as on the host) or in none.  Key =
+// slice * C + class; an empty slot holds S * C and sorts last.
+__global__ void k_term_keys(int n, const int* __restrict__ term_atom, const int* __restrict__ term_cls,
+                            const double* __restrict__ xyz, int S, int C, double origin, double dz,
+                            unsigned* __restrict__ keys, unsigned* __restrict__ vals, int* __restrict__ counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double z = xyz[3 * term_atom[i] + 2];
+  const unsigned none = (unsigned)S * (unsigned)C;
+  unsigned k[2] = {none, none};
+  const double f = floor((z - origin) / dz);
+  if (f > -4.0 && f < (double)S + 4.0) {  // false for NaN
+    const int s0 = (int)f;
+    int hit = 0;
+    for (int s = max(s0 - 2, 0); s <= min(s0 + 2, S - 1) && hit < 2; ++s) {
+      const double a = origin + dz * (double)s;
+      if (z >= a && z < a + dz) k[hit++] = (unsigned)s * (unsigned)C + (unsigned)term_cls[i];
+    }
+  }
+  for (int h = 0; h < 2; ++h) {
+    keys[2 * i + h] = k[h];
+    vals[2 * i + h] = (unsigned)i;
+    if (k[h] != none) atomicAdd(&counts[k[h]], 1);
+  }
+}
+
+__global__ void k_fill_i32(int n, int v, int* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_term_gather(int n, const unsigned* __restrict__ vals, const int* __restrict__ term_atom,
+                              const double* __restrict__ term_w, const double* __restrict__ xyz,
+                              double* __restrict__ tx, double* __restrict__ ty, double* __restrict__ wt) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = (int)vals[p];
+  const int a = term_atom[i];
+  tx[p] = xyz[3 * a];
+  ty[p] = xyz[3 * a + 1];
+  wt[p] = term_w[i];
+}
+
+}  // namespace
+
+void displace(hipStream_t st, int n, const double* xyz, const double* rms, uint64_t seed, int config, double lo,
+              double hi, const double* faces, int n_faces, double* out) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_displace, dim3((n + 255) / 256), dim3(256), 0, st, n, xyz, rms, seed, config, lo, hi, faces,
+                       n_faces, out);
+}
+
+void term_keys(hipStream_t st, int n, const int* term_atom, const int* term_cls, const double* xyz, int S, int C,
+               double origin, double dz, unsigned* keys, unsigned* vals, int* counts) {
+  const int nc = S * C;
+  hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, st, nc, 0, counts);
+  if (n > 0)
+    hipLaunchKernelGGL(k_term_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, term_atom, term_cls, xyz, S, C, origin,
+                       dz, keys, vals, counts);
+}
+
+size_t sort_pairs_temp_bytes(int n, int bits) {
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                           (const unsigned*)nullptr, (unsigned*)nullptr, n, 0, bits);
+  return bytes;
+}
+
+hipError_t sort_pairs(hipStream_t st, void* temp, size_t temp_bytes, const unsigned* keys_in, unsigned* keys_out,
+                      const unsigned* vals_in, unsigned* vals_out, int n, int bits) {
+  size_t bytes = temp_bytes;
+  return hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys_in, keys_out, vals_in, vals_out, n, 0, bits, st);
+}
+
+void term_gather(hipStream_t st, int n, const unsigned* vals, const int* term_atom, const double* term_w,
+                 const double* xyz, double* tx, double* ty, double* wt) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_term_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, vals, term_atom, term_w, xyz, tx, ty,
+                       wt);
+}
+
+}  // namespace totem

@@ -38,6 +38,7 @@ EXPORTS = (
     "totem_stem_preferred_batch", "totem_stem_autotune", "totem_hrtem_images", "totem_tcc_image", "totem_weighted_pattern",
     "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
     "totem_server_destroy",
+    "totem_specimen_create", "totem_specimen_transmission", "totem_specimen_positions", "totem_specimen_destroy",
 )
 
 TOTEM_RUN_GRAPH = 1
@@ -70,6 +71,13 @@ class PotentialDesc(ctypes.Structure):
         ("corr_info_matrix", _DP), ("cim_len", _I),
         ("kx", _DP), ("ky", _DP), ("s_2", _DP), ("aper_true", _DP),
         ("sigma", _D), ("parameter", _D),
+    ]
+
+
+class SpecimenDesc(ctypes.Structure):
+    _fields_ = [
+        ("pot", PotentialDesc), ("n_atoms", _I), ("xyz", _DP), ("rms", _DP), ("term_atom", _IP), ("n_terms", _I),
+        ("slices", _I), ("slice_origin", _D), ("slice_thickness", _D), ("mid", _IP), ("n_mid", _I),
     ]
 
 
@@ -136,6 +144,12 @@ def lib():
         L.totem_stem_autotune.restype = _I
         L.totem_stem_preferred_batch.argtypes = [_I]
         L.totem_stem_preferred_batch.restype = _I
+        L.totem_specimen_create.argtypes = [ctypes.POINTER(SpecimenDesc), ctypes.POINTER(_VP)]
+        L.totem_specimen_transmission.argtypes = [_VP, ctypes.c_ulonglong, _I, _VP, _VP]
+        L.totem_specimen_positions.argtypes = [_VP, ctypes.c_ulonglong, _I, _DP]
+        L.totem_specimen_destroy.argtypes = [_VP]
+        for name in ("create", "transmission", "positions", "destroy"):
+            getattr(L, "totem_specimen_" + name).restype = _I
         L.totem_server_create.argtypes = [ctypes.c_size_t, _I, _I, ctypes.POINTER(_VP)]
         L.totem_server_submit.argtypes = [_VP, ctypes.POINTER(StemJob), ctypes.POINTER(ctypes.c_longlong)]
         L.totem_server_step.argtypes = [_VP, ctypes.POINTER(_I)]
@@ -241,6 +255,21 @@ def build_transmission(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx,
                        ele_n_i_corr=None, corr_info_matrix=None, want_v=False, stream=None):
     """Build the transmission stack t (complex64 [S, Np, Np]) into device memory at t_dev_ptr.
     Returns (V_re, V_im) host arrays when want_v, else None."""
+    d, keep = _potential_desc(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx, ky, s_2, aper_true, sigma,
+                              parameter, absorp_n, atom_slice_i, proj_coff_mat_i, absorp_n_i, series_n_corr,
+                              ele_n_corr, series_n_i_corr, ele_n_i_corr, corr_info_matrix)
+    vre = vim = None
+    if want_v:
+        vre = np.zeros((slices, np_side, np_side))
+        vim = np.zeros((slices, np_side, np_side))
+    check(lib().totem_build_transmission(ctypes.byref(d), _VP(t_dev_ptr), _dp(vre), _dp(vim), _VP(stream)))
+    return (vre, vim) if want_v else None
+
+
+def _potential_desc(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx, ky, s_2, aper_true, sigma, parameter,
+                    absorp_n=None, atom_slice_i=None, proj_coff_mat_i=None, absorp_n_i=None, series_n_corr=None,
+                    ele_n_corr=None, series_n_i_corr=None, ele_n_i_corr=None, corr_info_matrix=None):
+    """A PotentialDesc and the host arrays it points into (keep them alive while it is used)."""
     keep = {}
 
     def p(name, a):
@@ -270,12 +299,7 @@ def build_transmission(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx,
     d.aper_true = p("at", aper_true)
     d.sigma = float(sigma)
     d.parameter = float(parameter)
-    vre = vim = None
-    if want_v:
-        vre = np.zeros((slices, np_side, np_side))
-        vim = np.zeros((slices, np_side, np_side))
-    check(lib().totem_build_transmission(ctypes.byref(d), _VP(t_dev_ptr), _dp(vre), _dp(vim), _VP(stream)))
-    return (vre, vim) if want_v else None
+    return d, keep
 
 
 def stem_job(n, np_side, slices, t_ptr, probes_ptr, n_probes, prop_ptr, aper_ptr, n_aper, depth, width_red,
@@ -434,3 +458,67 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class DeviceSpecimen:
+    """A specimen resident on the GPU that is displaced (frozen phonon) and sliced there per
+    configuration (``totem_specimen_*``).  ``rows`` holds every atom's coefficient rows as one slice
+    (``stem.specimen_rows``), ``term_atom`` the atom of each row (ele_n, ele_n_i, ele_n_corr order)."""
+
+    def __init__(self, paraflag, np_side, rows, term_atom, xyz, rms, kx, ky, s_2, aper_true, sigma, parameter,
+                 slices, slice_origin, slice_thickness, mid=(), corr_info_matrix=None):
+        d, keep = _potential_desc(paraflag, np_side, 1, rows["series_n"], rows["ele_n"], kx, ky, s_2, aper_true,
+                                  sigma, parameter, atom_slice_i=rows["series_n_i"], proj_coff_mat_i=rows["ele_n_i"],
+                                  series_n_corr=rows.get("series_n_corr"), ele_n_corr=rows.get("ele_n_corr"),
+                                  corr_info_matrix=corr_info_matrix)
+        sd = SpecimenDesc()
+        sd.pot = d
+        xyz = _f64(xyz).reshape(-1, 3)
+        keep["xyz"] = xyz
+        sd.n_atoms = xyz.shape[0]
+        sd.xyz = _dp(xyz)
+        if rms is not None:
+            rms = _f64(rms).reshape(-1)
+            if rms.shape[0] != sd.n_atoms:
+                raise ValueError("rms needs one entry per atom")
+        keep["rms"] = rms
+        sd.rms = _dp(rms)
+        ta = np.ascontiguousarray(np.asarray(term_atom, dtype=np.int32))
+        keep["ta"] = ta
+        sd.term_atom = ta.ctypes.data_as(_IP) if ta.size else None
+        sd.n_terms = int(ta.size)
+        sd.slices = int(slices)
+        sd.slice_origin = float(slice_origin)
+        sd.slice_thickness = float(slice_thickness)
+        mid = np.ascontiguousarray(np.asarray(mid, dtype=np.int32))
+        keep["mid"] = mid
+        sd.mid = mid.ctypes.data_as(_IP) if mid.size else None
+        sd.n_mid = int(mid.size)
+        h = _VP()
+        check(lib().totem_specimen_create(ctypes.byref(sd), ctypes.byref(h)))
+        self._h = h
+        self.n_atoms = sd.n_atoms
+        self.np_side = int(np_side)
+        self.slices = int(slices)
+
+    def transmission(self, seed, config, t_dev_ptr, stream=None):
+        """t (complex64 [S, Np, Np]) of configuration ``config`` into device memory at t_dev_ptr."""
+        check(lib().totem_specimen_transmission(self._h, ctypes.c_ulonglong(seed), int(config), _VP(t_dev_ptr),
+                                                _VP(stream)))
+
+    def positions(self, seed, config):
+        """Displaced positions [n_atoms, 3] of configuration ``config``."""
+        out = np.zeros((self.n_atoms, 3))
+        check(lib().totem_specimen_positions(self._h, ctypes.c_ulonglong(seed), int(config), _dp(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            check(lib().totem_specimen_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -6,7 +6,7 @@ and its marshalling script ``cuda_stem.m:1-107``: it builds the specimen rows
 41 mex arguments, and calls the engine variant chosen by ``paraflag``
 ('l' -> waveCudalobato, 'p'/'n' -> waveCudaPengAndCorr).
 """
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -97,6 +97,54 @@ def phonon_snapshots(cfg, count, sigma_by_z, seed=0):
         xs[:, 6] = 0.0  # frozen phonon only: no Debye-Waller attenuation on top of the displacements
         out.append(xs)
     return out
+
+
+def specimen_rows(cfg, atoms=None):
+    """Every atom's coefficient rows as ONE slice, for slicing on the device.
+
+    Returns (rows, term_atom, corr_matrix): rows holds series_n, ele_n, series_n_i, ele_n_i,
+    series_n_corr, ele_n_corr exactly as :func:`prepare` builds them for a single slice that holds
+    the whole specimen (multiceng = 0, so no erf weighting); term_atom gives the atom (row of
+    ``atoms``) of each row in the order ele_n, ele_n_i, ele_n_corr -- the order
+    ``getpotentialpara_lotabo_peng.m`` and ``getpotentialpara_corr_peng.m`` emit them in (neutral,
+    then fully ionic, then mixed; corrections for neutral atoms)."""
+    x = np.asarray(cfg.atoms if atoms is None else atoms, dtype=np.float64)
+    if len(x) == 0:
+        raise ValueError("specimen has no atoms")
+    z = x[:, 3]
+    one = replace(cfg, atoms=x, n_slices=1, slice_origin=float(z.min()) - 1.0,
+                  slice_thickness=float(z.max() - z.min()) + 2.0, multiceng=0, mid_slices=())
+    inp = prepare(one, x, optics_too=False)
+    ion = x[:, 5]
+    ta = [np.nonzero(ion == 0)[0], np.nonzero(ion == 1)[0], np.nonzero((ion > 0) & (ion < 1))[0]]
+    if cfg.paraflag == "n":
+        ta.append(np.nonzero(ion == 0)[0])
+    flag = cfg.paraflag
+    rows = dict(series_n=np.asarray(inp.series_n[:1], dtype=np.float64), ele_n=inp.ele_n,
+                series_n_i=np.full(1, float(len(inp.ele_n_i))),
+                ele_n_i=inp.ele_n_i if len(inp.ele_n_i) else np.zeros((0, 26 if flag == "l" else 25)))
+    if flag == "n":
+        rows["series_n_corr"] = np.asarray(inp.series_n_corr[:1], dtype=np.float64)
+        rows["ele_n_corr"] = inp.ele_n_corr
+    return rows, np.concatenate(ta).astype(np.int32), inp.corr_matrix
+
+
+def device_specimen(cfg, sigma_by_z=None, rms=None, atoms=None):
+    """The specimen resident on the GPU (:class:`totem_amd.engine.DeviceSpecimen`), displaced and
+    sliced there per frozen-phonon configuration.  Displacement rms per atom from ``sigma_by_z``
+    ({Z: rms}, Angstrom; the Debye-Waller column is then cleared as in :func:`phonon_snapshots`) or
+    ``rms`` directly; neither gives a static specimen."""
+    from . import engine
+    x = np.array(cfg.atoms if atoms is None else atoms, dtype=np.float64)
+    if sigma_by_z is not None:
+        rms = np.array([sigma_by_z.get(float(zz), sigma_by_z.get(int(zz), 0.0)) for zz in x[:, 0]])
+        x[:, 6] = 0.0
+    rows, term_atom, cim = specimen_rows(cfg, x)
+    gx, gy, s2 = optics.reciprocal_grids(cfg.np_side, cfg.dx)
+    return engine.DeviceSpecimen(cfg.paraflag, cfg.np_side, rows, term_atom, x[:, 1:4], rms, gx, gy, s2,
+                                 np.ones_like(s2), optics.interaction_sigma(cfg.kv),
+                                 optics.potential_parameter(cfg.np_side, cfg.dx), cfg.n_slices, cfg.slice_origin,
+                                 cfg.slice_thickness, tuple(cfg.mid_slices), cim)
 
 
 def prepare(cfg, atoms=None, optics_too=True):
