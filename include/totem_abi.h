@@ -163,9 +163,9 @@ void totem_kernel_times(int reset, double *rows_ms, long *rows_n, double *cols_m
 
 /* ----------------------------------------------- specimen on the device
  * Frozen-phonon displacement and slicing on the GPU (the host steps before
- * myProjectfun in the reference: randshiftpos.m:1-44 and
- * CalAllEquilent20210223_lobato_peng.m:23-57 with multiceng = 0,
- * CalAllEquilent20210305_peng_corr.m:25-70 without half-splitting).  `pot`
+ * myProjectfun in the reference: randshiftpos.m:1-44,
+ * CalAllEquilent20210223_lobato_peng.m:23-57,213-226 and
+ * CalAllEquilent20210305_peng_corr.m:25-70).  `pot`
  * holds every atom's coefficient rows as one slice (slices = 1; grids, sigma,
  * parameter, correction tables as for totem_build_transmission); term_atom maps
  * each of its rows, in the order proj_coff_mat, proj_coff_mat_i, ele_n_corr,
@@ -180,8 +180,11 @@ typedef struct totem_specimen_desc {
   int slices;             /* S */
   double slice_origin;    /* st[0]; slice i is st[i] <= z < st[i] + slice_thickness, st[i] = st[0] + i dz */
   double slice_thickness;
-  const int *mid;         /* [n_mid] 1-based slices whose lower face st[m-1] + dz atoms must not cross */
+  const int *mid;         /* [n_mid] 1-based slices whose upper face st[m-1] + dz atoms must not cross */
   int n_mid;
+  double multiceng;       /* Peng: atoms within multiceng * dz of a face also feed the neighbouring
+                             slices, erf-weighted per Gaussian term; corrections within 0.015 A of a
+                             face are shared half and half when multiceng >= 1.  Lobato: ignored (0). */
 } totem_specimen_desc;
 typedef struct totem_specimen totem_specimen;
 int totem_specimen_create(const totem_specimen_desc *desc, totem_specimen **out);

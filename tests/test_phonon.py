@@ -1,12 +1,14 @@
 """Frozen-phonon displacement and specimen slicing on the device (randshiftpos.m:1-44,
-CalAllEquilent20210223_lobato_peng.m:23-57 with multiceng = 0).
+CalAllEquilent20210223_lobato_peng.m:23-57,213-226, CalAllEquilent20210305_peng_corr.m:25-70).
 
 CPU: the oracle's Philox4x32-10 against the Random123 known-answer vectors; the normals' moments;
-accepted positions obey the slab and mid-slice faces; slicing at and next to faces; the one-slice
-rows the device specimen starts from sit at their atoms.
+accepted positions obey the slab and mid-slice faces; slicing at and next to faces; correction rows
+near faces shared half and half; the one-slice rows the device specimen starts from sit at their atoms.
 GPU: displaced positions equal the oracle's (same generator, <= 1e-12 Angstrom); the transmission
 built from the device-sliced specimen is bit-identical to totem_build_transmission on the rows the
-host slices from the same positions, and within 1e-6 relative L2 of the FP64 oracle potential.
+host slices from the same positions, and within 1e-6 relative L2 of the FP64 oracle potential; with
+multiceng > 0 (erf-weighted rows, shared corrections) within 1e-6 of the oracle and of the host-sliced
+build.
 """
 import numpy as np
 import pytest
@@ -75,6 +77,22 @@ def test_slicing_at_faces_goes_up():
     all_n, all_n_i = specimen.atom_parameters(atoms, "l")
     _, _, series, _ = specimen.slice_atoms(all_n, all_n_i, dz, st, 0, "l", 0.0)
     np.testing.assert_array_equal(series, [len(m) for m in mem])
+
+
+def test_correction_rows_near_faces_are_shared_half_and_half():
+    dz, S = 1.5, 4
+    st = -0.2 + dz * np.arange(S)
+    z = [st[2] + 0.005, st[1] + dz - 0.005, st[0] + 0.005, st[3] + dz - 0.005, st[2] + 0.7]
+    x = np.zeros((len(z), 8))
+    x[:, 0], x[:, 1], x[:, 2], x[:, 3], x[:, 7] = 8, np.arange(len(z)), 0.5, z, 0.8
+    cn = specimen.corr_parameters(x)
+    e, _, series = specimen.slice_corr(cn, dz, st, 1)
+    # [x y B occ*w cls]: atoms 0 and 1 split over slices (1, 2), the outer faces and atom 4 do not
+    got = sorted((int(r[0]), float(r[3])) for r in e)
+    assert got == [(0, 0.4), (0, 0.4), (1, 0.4), (1, 0.4), (2, 0.8), (3, 0.8), (4, 0.8)]
+    np.testing.assert_array_equal(series, [1, 2, 3, 1])
+    e0, _, series0 = specimen.slice_corr(cn, dz, st, 0)     # multiceng = 0: no sharing
+    assert sorted(float(r[3]) for r in e0) == [0.8] * 5
 
 
 def _ionic_perovskite(S, nx=2, seed=3):
@@ -234,3 +252,53 @@ def test_frozen_phonon_configuration_at_full_size(engine):
           " host slicing %.2f s + build from host rows %.1f ms" % (t1 - t0, (t3 - t2) / 3 * 1e3, (t2 - t1) * 1e3,
                                                                   t5 - t4, (t6 - t5 - (t5 - t4)) * 1e3))
     assert np.array_equal(td, th)
+
+
+def _near_face_case(flag, multiceng):
+    """A perovskite whose atoms partly sit within 0.01 A of slice faces (corrections shared) and within
+    multiceng * dz of them (erf-weighted rows); mixed and fully ionic atoms included."""
+    cfg, sig = _case(flag)
+    x = cfg.atoms.copy()
+    st = stem.slice_boundaries(cfg)
+    dz = cfg.slice_thickness
+    rng = np.random.default_rng(5)
+    pick = rng.choice(len(x), 12, replace=False)
+    for j, a in enumerate(pick):
+        i = 1 + j % (cfg.n_slices - 2)
+        x[a, 3] = st[i] + 0.004 if j % 2 == 0 else st[i] + dz - 0.004
+    from dataclasses import replace
+    return replace(cfg, atoms=x, multiceng=multiceng), sig
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flag,multiceng", [("p", 1.0), ("n", 1.0), ("p", 0.5), ("n", 0.5)])
+def test_device_erf_weighted_slicing_matches_oracle(engine, flag, multiceng):
+    import torch
+    from oracle import potential as opot
+    print("case specimen slicing flag=%s multiceng=%g" % (flag, multiceng), flush=True)
+    cfg, sig = _near_face_case(flag, multiceng)
+    S, npx = cfg.n_slices, cfg.np_side
+    t = torch.empty((S, npx, npx), dtype=torch.complex64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    for label, sigma_by_z in (("static", None), ("displaced", sig)):
+        with stem.device_specimen(cfg, sigma_by_z=sigma_by_z) as sp:
+            sp.transmission(42, 1, t.data_ptr(), stream=st)
+            torch.cuda.synchronize()
+            td = t.cpu().numpy().astype(np.complex128)
+            xd = cfg.atoms.copy()
+            xd[:, 1:4] = sp.positions(42, 1)
+        if sigma_by_z is not None:
+            xd[:, 6] = 0.0
+        th, inp = _host_transmission(engine, cfg, xd)
+        kw = {}
+        if flag == "n":
+            kw = dict(ele_n_corr=inp.ele_n_corr, corr_matrix=inp.corr_matrix, series_n_corr=inp.series_n_corr)
+        _, to = opot.potential_slices(npx, inp.ele_n, None, inp.ele_n_i, None, inp.series_n, inp.series_n_i,
+                                      inp.s2, inp.gx, inp.gy, inp.sigma, inp.parameter, inp.aper_true, flag, **kw)
+        e_o = max(float(np.linalg.norm(td[s] - to[s]) / np.linalg.norm(to[s])) for s in range(S))
+        e_h = max(float(np.linalg.norm(td[s] - th[s]) / np.linalg.norm(th[s])) for s in range(S))
+        n_rows = len(inp.ele_n) + len(inp.ele_n_i)
+        print("  %s: host rows %d (atoms %d), corr rows %s; max t rel L2 vs oracle %.3e, vs host-sliced build %.3e"
+              % (label, n_rows, len(xd), len(inp.ele_n_corr) if flag == "n" else "-", e_o, e_h))
+        assert n_rows > len(xd)
+        assert e_o <= 1e-6 and e_h <= 1e-6

@@ -424,6 +424,10 @@ static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev
 
 // ------------------------------------------------------- specimen on the device
 
+// Correction terms this close (Angstrom) to a slice face are shared with the
+// neighbouring slice (CalAllEquilent20210305_peng_corr.m:41-58, discon).
+constexpr double kCorrDiscon = 0.015;
+
 }  // namespace totem
 
 // Resident specimen: static atoms and per-term data on the device; per phonon
@@ -432,11 +436,12 @@ static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev
 struct totem_specimen {
   std::mutex mu;
   int dev = 0;
-  int np = 0, S = 0, C = 0, n_atoms = 0, n_terms = 0, n_faces = 0, bits = 1;
-  double origin = 0, dz = 0, lo = 0, hi = 0, sigma = 0, parameter = 0;
+  int np = 0, S = 0, Ct = 0, n_atoms = 0, n_terms = 0, n_faces = 0, bits = 1, K = 2, split = 0;
+  double origin = 0, dz = 0, lo = 0, hi = 0, sigma = 0, parameter = 0, m = 0;
   bool has_rms = false;
   size_t sort_bytes = 0;
-  totem::DevBuf xyz, rms, pos, faces, term_atom, term_cls, term_w, keys, keys2, vals, vals2, counts, sort_tmp;
+  totem::DevBuf xyz, rms, pos, faces, term_atom, term_cls, term_w, term_mode, sub_base, keys, keys2, vals, vals2,
+      wslot, counts, sort_tmp;
   totem::DevBuf tx, ty, wt, segs, classes, kx, ky, s2, aper, corr, ex, ey;
   std::vector<int> h_counts;
   std::vector<totem::PotSeg> h_segs;
@@ -452,14 +457,40 @@ static totem_specimen* specimen_create_impl(const totem_specimen_desc& d) {
   require(d.n_atoms >= 0 && (d.n_atoms == 0 || d.xyz), "specimen: xyz missing");
   require(d.n_terms >= 0 && (d.n_terms == 0 || d.term_atom), "specimen: term_atom missing");
   require(d.n_mid >= 0 && (d.n_mid == 0 || d.mid), "specimen: mid missing");
+  require(d.multiceng >= 0 && std::isfinite(d.multiceng), "specimen: multiceng must be finite and >= 0");
   PotentialPlan P = plan_potential(d.pot);
   const auto& terms = P.slices[0];
   require((long)terms.size() == (long)d.n_terms,
           "specimen: n_terms = " + std::to_string(d.n_terms) + " but pot holds " + std::to_string(terms.size()) +
               " rows");
-  const int np = P.np, S = d.slices, C = std::max<int>(1, (int)P.classes.size());
-  require((long)S * C < (1L << 30), "specimen: slices x form-factor classes exceeds 2^30");
-  std::vector<int> t_atom(d.term_atom, d.term_atom + d.n_terms), t_cls(d.n_terms);
+  const int np = P.np, S = d.slices, C = (int)P.classes.size();
+  // Lobato slicing is single-slice whatever multiceng says (ToTEM_submit.m:874-877).
+  const double m = d.pot.paraflag == 'l' ? 0.0 : d.multiceng;
+  // One-term classes for erf-weighted rows: class c's Gaussian q alone at sub_base[c] + q.
+  std::vector<int> sub_base(std::max(C, 1), -1);
+  std::vector<PotClass> classes = P.classes;
+  int nq_max = 0;
+  if (m > 0)
+    for (int c = 0; c < C; ++c) {
+      const PotClass k = P.classes[c];
+      if (k.kind != kPeng && k.kind != kPengIon) continue;
+      require(!k.has_abs, "specimen: absorptive coefficients with multiceng > 0 are not supported",
+              TOTEM_ERR_UNSUPPORTED);
+      const int nq = k.kind == kPengIon ? 10 : 5;
+      nq_max = std::max(nq_max, nq);
+      sub_base[c] = (int)classes.size();
+      for (int q = 0; q < nq; ++q) {
+        PotClass one = k;
+        for (int j = 0; j < 5; ++j) {
+          if (j != q) one.p1[2 * j] = 0.0;
+          if (j + 5 != q) one.p2[2 * j] = 0.0;
+        }
+        classes.push_back(one);
+      }
+    }
+  const int Ct = std::max<int>(1, (int)classes.size());
+  require((long)S * Ct < (1L << 30), "specimen: slices x form-factor classes exceeds 2^30");
+  std::vector<int> t_atom(d.term_atom, d.term_atom + d.n_terms), t_cls(d.n_terms), t_mode(d.n_terms);
   std::vector<double> t_w(d.n_terms);
   for (int i = 0; i < d.n_terms; ++i) {
     const int a = t_atom[i];
@@ -468,7 +499,13 @@ static totem_specimen* specimen_create_impl(const totem_specimen_desc& d) {
             "specimen: row " + std::to_string(i) + " does not sit at its atom's x, y");
     t_cls[i] = terms[i].cls;
     t_w[i] = terms[i].w;
+    const int kind = P.classes[terms[i].cls].kind;
+    t_mode[i] = kind == kPeng ? 1 : kind == kPengIon ? 2 : (kind == kCorr || kind == kCorrIon) ? 3 : 0;
   }
+  // key slots per term: two slices at a rounding face, each split in two for
+  // corrections, or every slice of the widened range times the Gaussian terms
+  const int K = std::max(4, m > 0 ? ((int)std::floor(2.0 * m + 1.0) + 2) * nq_max : 0);
+  require((long)d.n_terms * K < (1L << 31), "specimen: too many terms for the slot index");
   std::vector<double> faces;
   for (int m = 0; m < d.n_mid; ++m) {
     require(d.mid[m] >= 1 && d.mid[m] <= S, "specimen: mid slice out of range");
@@ -483,7 +520,10 @@ static totem_specimen* specimen_create_impl(const totem_specimen_desc& d) {
   HIP_OK(hipGetDevice(&sp->dev));
   sp->np = np;
   sp->S = S;
-  sp->C = C;
+  sp->Ct = Ct;
+  sp->K = K;
+  sp->m = m;
+  sp->split = (m >= 1.0 && d.slice_thickness > 2.0 * kCorrDiscon) ? 1 : 0;
   sp->n_atoms = d.n_atoms;
   sp->n_terms = d.n_terms;
   sp->n_faces = (int)faces.size();
@@ -494,16 +534,18 @@ static totem_specimen* specimen_create_impl(const totem_specimen_desc& d) {
   sp->sigma = d.pot.sigma;
   sp->parameter = d.pot.parameter;
   sp->has_rms = d.rms != nullptr;
-  while ((1L << sp->bits) <= (long)S * C) ++sp->bits;
+  while ((1L << sp->bits) <= (long)S * Ct) ++sp->bits;
   hipStream_t st = lib_stream();
-  const size_t np2 = (size_t)np * np, n2 = 2 * (size_t)std::max(d.n_terms, 1);
+  const size_t np2 = (size_t)np * np, n2 = (size_t)K * std::max(d.n_terms, 1);
   upload(sp->xyz, xyz, st);
   upload(sp->rms, rms, st);
   upload(sp->faces, faces, st);
   upload(sp->term_atom, t_atom, st);
   upload(sp->term_cls, t_cls, st);
   upload(sp->term_w, t_w, st);
-  upload(sp->classes, P.classes, st);
+  upload(sp->term_mode, t_mode, st);
+  upload(sp->sub_base, sub_base, st);
+  upload(sp->classes, classes, st);
   upload(sp->kx, P.kx1, st);
   upload(sp->ky, P.ky1, st);
   reserve(sp->pos, std::max<size_t>(3 * (size_t)d.n_atoms, 1) * sizeof(double));
@@ -517,12 +559,12 @@ static totem_specimen* specimen_create_impl(const totem_specimen_desc& d) {
                           hipMemcpyHostToDevice, st));
   }
   for (DevBuf* b : {&sp->keys, &sp->keys2, &sp->vals, &sp->vals2}) reserve(*b, n2 * sizeof(unsigned));
-  reserve(sp->counts, (size_t)S * C * sizeof(int));
-  for (DevBuf* b : {&sp->tx, &sp->ty, &sp->wt}) reserve(*b, n2 * sizeof(double));
-  reserve(sp->segs, std::max<size_t>((size_t)S * C, 1) * sizeof(PotSeg));
+  reserve(sp->counts, (size_t)S * Ct * sizeof(int));
+  for (DevBuf* b : {&sp->wslot, &sp->tx, &sp->ty, &sp->wt}) reserve(*b, n2 * sizeof(double));
+  reserve(sp->segs, std::max<size_t>((size_t)S * Ct, 1) * sizeof(PotSeg));
   sp->sort_bytes = sort_pairs_temp_bytes((int)n2, sp->bits);
   reserve(sp->sort_tmp, sp->sort_bytes);
-  sp->h_counts.resize((size_t)S * C);
+  sp->h_counts.resize((size_t)S * Ct);
   HIP_OK(hipStreamSynchronize(st));
   return sp.release();
 }
@@ -535,10 +577,30 @@ static void specimen_positions_impl(totem_specimen& sp, uint64_t seed, int confi
 
 static void specimen_transmission_impl(totem_specimen& sp, uint64_t seed, int config, float2* t_dev,
                                        hipStream_t st) {
-  const int S = sp.S, C = sp.C, n2 = 2 * sp.n_terms;
+  const int S = sp.S, C = sp.Ct, n2 = sp.K * sp.n_terms;
   specimen_positions_impl(sp, seed, config, st);
-  term_keys(st, sp.n_terms, sp.term_atom.as<int>(), sp.term_cls.as<int>(), sp.pos.as<double>(), S, C, sp.origin,
-            sp.dz, sp.keys.as<unsigned>(), sp.vals.as<unsigned>(), sp.counts.as<int>());
+  TermKeyArgs ka;
+  ka.n = sp.n_terms;
+  ka.K = sp.K;
+  ka.S = S;
+  ka.Ct = C;
+  ka.split = sp.split;
+  ka.term_atom = sp.term_atom.as<int>();
+  ka.term_cls = sp.term_cls.as<int>();
+  ka.term_mode = sp.term_mode.as<int>();
+  ka.sub_base = sp.sub_base.as<int>();
+  ka.term_w = sp.term_w.as<double>();
+  ka.xyz = sp.pos.as<double>();
+  ka.classes = sp.classes.as<PotClass>();
+  ka.origin = sp.origin;
+  ka.dz = sp.dz;
+  ka.m = sp.m;
+  ka.discon = kCorrDiscon;
+  ka.keys = sp.keys.as<unsigned>();
+  ka.vals = sp.vals.as<unsigned>();
+  ka.wslot = sp.wslot.as<double>();
+  ka.counts = sp.counts.as<int>();
+  term_keys(st, ka);
   HIP_OK(hipGetLastError());
   if (n2 > 0)
     HIP_OK(sort_pairs(st, sp.sort_tmp.p, sp.sort_bytes, sp.keys.as<unsigned>(), sp.keys2.as<unsigned>(),
@@ -569,7 +631,7 @@ static void specimen_transmission_impl(totem_specimen& sp, uint64_t seed, int co
   if (!sp.h_segs.empty())
     HIP_OK(hipMemcpyAsync(sp.segs.p, sp.h_segs.data(), sp.h_segs.size() * sizeof(PotSeg), hipMemcpyHostToDevice,
                           st));
-  term_gather(st, (int)total, sp.vals2.as<unsigned>(), sp.term_atom.as<int>(), sp.term_w.as<double>(),
+  term_gather(st, (int)total, sp.K, sp.vals2.as<unsigned>(), sp.term_atom.as<int>(), sp.wslot.as<double>(),
               sp.pos.as<double>(), sp.tx.as<double>(), sp.ty.as<double>(), sp.wt.as<double>());
   HIP_OK(hipGetLastError());
   pd.np = sp.np;

@@ -40,16 +40,26 @@ void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double
                 const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
                 double parameter, int np, float2* vshift);
 
-// Specimen slicing on the device (specimen.hip).  term_keys clears counts[S * C],
-// then writes keys/vals[2 n] (two slots per term) and counts terms per key.
+// Specimen slicing on the device (specimen.hip).
 void displace(hipStream_t st, int n, const double* xyz, const double* rms, uint64_t seed, int config, double lo,
               double hi, const double* faces, int n_faces, double* out);
-void term_keys(hipStream_t st, int n, const int* term_atom, const int* term_cls, const double* xyz, int S, int C,
-               double origin, double dz, unsigned* keys, unsigned* vals, int* counts);
+// Clears counts[S * Ct], then writes keys / vals / wslot[n * K] (K slots per
+// term) and counts the slots per (slice, class) key.
+struct TermKeyArgs {
+  int n = 0, K = 2, S = 0, Ct = 0, split = 0;
+  const int *term_atom = nullptr, *term_cls = nullptr, *term_mode = nullptr, *sub_base = nullptr;
+  const double *term_w = nullptr, *xyz = nullptr;
+  const PotClass* classes = nullptr;
+  double origin = 0, dz = 0, m = 0, discon = 0;
+  unsigned *keys = nullptr, *vals = nullptr;
+  double* wslot = nullptr;
+  int* counts = nullptr;
+};
+void term_keys(hipStream_t st, const TermKeyArgs& a);
 size_t sort_pairs_temp_bytes(int n, int bits);  // stable radix sort of (key, value) pairs on bits [0, bits)
 hipError_t sort_pairs(hipStream_t st, void* temp, size_t temp_bytes, const unsigned* keys_in, unsigned* keys_out,
                       const unsigned* vals_in, unsigned* vals_out, int n, int bits);
-void term_gather(hipStream_t st, int n, const unsigned* vals, const int* term_atom, const double* term_w,
+void term_gather(hipStream_t st, int n, int K, const unsigned* vals, const int* term_atom, const double* wslot,
                  const double* xyz, double* tx, double* ty, double* wt);
 
 }  // namespace totem

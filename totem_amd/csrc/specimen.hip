@@ -3,7 +3,7 @@
 // The reference displaces atoms on the host (randshiftpos.m:1-44: Gaussian
 // per axis, redrawn while an atom leaves the slab or crosses a mid-slice face)
 // and assigns them to slices on the host (CalAllEquilent20210223_lobato_peng.m:
-// 23-57 with multiceng = 0: slice i holds st[i] <= z < st[i] + dz).  Here both
+// 23-57, 213-226; CalAllEquilent20210305_peng_corr.m:25-70).  Here both
 // run per phonon configuration on the device, on terms that carry their atom
 // and form-factor class: displaced positions -> (slice, class) keys -> stable
 // radix sort -> term arrays grouped by slice and class, the layout the
@@ -89,31 +89,102 @@ __global__ void k_displace(int n, const double* __restrict__ xyz, const double* 
   out[3 * a + 2] = pz;
 }
 
-// Two key slots per term: slice i holds st[i] <= z < st[i] + dz with
+// Up to K key slots per term.  Slice i holds st[i] <= z < st[i] + dz with
 // st[i] = origin + dz * i, the host's test, so an atom within rounding of a
 // face can sit in two adjacent slices (as on the host) or in none.  Key =
-// slice * C + class; an empty slot holds S * C and sorts last.
-__global__ void k_term_keys(int n, const int* __restrict__ term_atom, const int* __restrict__ term_cls,
-                            const double* __restrict__ xyz, int S, int C, double origin, double dz,
-                            unsigned* __restrict__ keys, unsigned* __restrict__ vals, int* __restrict__ counts) {
+// slice * Ct + class; an empty slot holds S * Ct and sorts last.  Per slot the
+// weight (occupancy times the slice's share) goes to wslot.
+//
+// mode kSplitCorr (correction terms, CalAllEquilent20210305_peng_corr.m:41-58):
+// with split set, a term within discon of its slice's lower (upper) face gives
+// half its weight to the slice below (above), the outermost faces excepted.
+// mode kErf5 / kErf10 (Peng rows, CalAllEquilent20210223_lobato_peng.m:23-57,
+// 213-226) with m > 0: every slice whose range widened by m dz on both sides
+// holds the atom gets it; where a face is nearer than m dz the Gaussian term q
+// is weighted by |erf(2 pi z2 / sqrt(b_q)) - erf(2 pi z1 / sqrt(b_q))| / 2 and
+// goes to the one-term class sub_base[class] + q.
+constexpr int kPlain = 0, kErf5 = 1, kErf10 = 2, kSplitCorr = 3;
+
+__device__ __forceinline__ double erf_share(double b, double z1, double z2) {
+  const double sb = sqrt(b);
+  return fabs(0.5 * erf(2.0 * M_PI * z2 / sb) - 0.5 * erf(2.0 * M_PI * z1 / sb));
+}
+
+__global__ void k_term_keys(int n, int K, const int* __restrict__ term_atom, const int* __restrict__ term_cls,
+                            const double* __restrict__ term_w, const int* __restrict__ term_mode,
+                            const PotClass* __restrict__ classes, const int* __restrict__ sub_base,
+                            const double* __restrict__ xyz, int S, int Ct, double origin, double dz, double m,
+                            int split, double discon, unsigned* __restrict__ keys, unsigned* __restrict__ vals,
+                            double* __restrict__ wslot, int* __restrict__ counts) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double z = xyz[3 * term_atom[i] + 2];
-  const unsigned none = (unsigned)S * (unsigned)C;
-  unsigned k[2] = {none, none};
+  const int cls = term_cls[i], mode = term_mode[i];
+  const double w = term_w[i];
+  const unsigned none = (unsigned)S * (unsigned)Ct;
+  const size_t base = (size_t)i * K;
+  int slot = 0;
+  auto emit = [&](int s, int c, double wt) {
+    if (slot >= K) return;
+    const unsigned key = (unsigned)s * (unsigned)Ct + (unsigned)c;
+    keys[base + slot] = key;
+    vals[base + slot] = (unsigned)(base + slot);
+    wslot[base + slot] = wt;
+    atomicAdd(&counts[key], 1);
+    ++slot;
+  };
   const double f = floor((z - origin) / dz);
-  if (f > -4.0 && f < (double)S + 4.0) {  // false for NaN
+  const bool erf_mode = (mode == kErf5 || mode == kErf10) && m > 0.0;
+  if (erf_mode) {
+    if (f > -m - 4.0 && f < (double)S + m + 4.0) {  // false for NaN
+      const int nq = mode == kErf10 ? 10 : 5;
+      const double md = m * dz;
+      const int s_lo = max((int)floor(f - m) - 2, 0), s_hi = min((int)ceil(f + m) + 2, S - 1);
+      for (int s = s_lo; s <= s_hi; ++s) {
+        const double st = origin + dz * (double)s;
+        if (!(z >= st - md && z < st + md + dz)) continue;
+        double z1 = z - (st + dz), z2 = z - st;
+        if (s == 0) z2 = INFINITY;
+        if (s == S - 1) z1 = -INFINITY;
+        if (fabs(z2) >= md) z2 = INFINITY;
+        if (fabs(z1) >= md) z1 = -INFINITY;
+        if (z2 == INFINITY && z1 == -INFINITY) {
+          emit(s, cls, w);  // every share is 1: the atom's own class
+          continue;
+        }
+        const PotClass& k = classes[cls];
+        for (int q = 0; q < nq; ++q) {
+          const double b = q < 5 ? k.p1[2 * q + 1] : k.p2[2 * (q - 5) + 1];
+          emit(s, sub_base[cls] + q, w * erf_share(b, z1, z2));
+        }
+      }
+    }
+  } else if (f > -4.0 && f < (double)S + 4.0) {
     const int s0 = (int)f;
     int hit = 0;
     for (int s = max(s0 - 2, 0); s <= min(s0 + 2, S - 1) && hit < 2; ++s) {
-      const double a = origin + dz * (double)s;
-      if (z >= a && z < a + dz) k[hit++] = (unsigned)s * (unsigned)C + (unsigned)term_cls[i];
+      const double st = origin + dz * (double)s;
+      if (!(z >= st && z < st + dz)) continue;
+      ++hit;
+      if (mode == kSplitCorr && split) {
+        if (z - st < discon && s > 0) {
+          emit(s, cls, w * 0.5);
+          emit(s - 1, cls, w * 0.5);
+          continue;
+        }
+        if (z - st - dz > -discon && s < S - 1) {
+          emit(s, cls, w * 0.5);
+          emit(s + 1, cls, w * 0.5);
+          continue;
+        }
+      }
+      emit(s, cls, w);
     }
   }
-  for (int h = 0; h < 2; ++h) {
-    keys[2 * i + h] = k[h];
-    vals[2 * i + h] = (unsigned)i;
-    if (k[h] != none) atomicAdd(&counts[k[h]], 1);
+  for (; slot < K; ++slot) {
+    keys[base + slot] = none;
+    vals[base + slot] = (unsigned)(base + slot);
+    wslot[base + slot] = 0.0;
   }
 }
 
@@ -122,16 +193,16 @@ __global__ void k_fill_i32(int n, int v, int* __restrict__ p) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_term_gather(int n, const unsigned* __restrict__ vals, const int* __restrict__ term_atom,
-                              const double* __restrict__ term_w, const double* __restrict__ xyz,
+__global__ void k_term_gather(int n, int K, const unsigned* __restrict__ vals, const int* __restrict__ term_atom,
+                              const double* __restrict__ wslot, const double* __restrict__ xyz,
                               double* __restrict__ tx, double* __restrict__ ty, double* __restrict__ wt) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const int i = (int)vals[p];
-  const int a = term_atom[i];
+  const unsigned slot = vals[p];
+  const int a = term_atom[slot / (unsigned)K];
   tx[p] = xyz[3 * a];
   ty[p] = xyz[3 * a + 1];
-  wt[p] = term_w[i];
+  wt[p] = wslot[slot];
 }
 
 }  // namespace
@@ -143,13 +214,13 @@ void displace(hipStream_t st, int n, const double* xyz, const double* rms, uint6
                        n_faces, out);
 }
 
-void term_keys(hipStream_t st, int n, const int* term_atom, const int* term_cls, const double* xyz, int S, int C,
-               double origin, double dz, unsigned* keys, unsigned* vals, int* counts) {
-  const int nc = S * C;
-  hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, st, nc, 0, counts);
-  if (n > 0)
-    hipLaunchKernelGGL(k_term_keys, dim3((n + 255) / 256), dim3(256), 0, st, n, term_atom, term_cls, xyz, S, C, origin,
-                       dz, keys, vals, counts);
+void term_keys(hipStream_t st, const TermKeyArgs& a) {
+  const int nc = a.S * a.Ct;
+  hipLaunchKernelGGL(k_fill_i32, dim3((nc + 255) / 256), dim3(256), 0, st, nc, 0, a.counts);
+  if (a.n > 0)
+    hipLaunchKernelGGL(k_term_keys, dim3((a.n + 255) / 256), dim3(256), 0, st, a.n, a.K, a.term_atom, a.term_cls,
+                       a.term_w, a.term_mode, a.classes, a.sub_base, a.xyz, a.S, a.Ct, a.origin, a.dz, a.m, a.split,
+                       a.discon, a.keys, a.vals, a.wslot, a.counts);
 }
 
 size_t sort_pairs_temp_bytes(int n, int bits) {
@@ -165,11 +236,11 @@ hipError_t sort_pairs(hipStream_t st, void* temp, size_t temp_bytes, const unsig
   return hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys_in, keys_out, vals_in, vals_out, n, 0, bits, st);
 }
 
-void term_gather(hipStream_t st, int n, const unsigned* vals, const int* term_atom, const double* term_w,
+void term_gather(hipStream_t st, int n, int K, const unsigned* vals, const int* term_atom, const double* wslot,
                  const double* xyz, double* tx, double* ty, double* wt) {
   if (n > 0)
-    hipLaunchKernelGGL(k_term_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, vals, term_atom, term_w, xyz, tx, ty,
-                       wt);
+    hipLaunchKernelGGL(k_term_gather, dim3((n + 255) / 256), dim3(256), 0, st, n, K, vals, term_atom, wslot, xyz, tx,
+                       ty, wt);
 }
 
 }  // namespace totem

@@ -78,6 +78,7 @@ class SpecimenDesc(ctypes.Structure):
     _fields_ = [
         ("pot", PotentialDesc), ("n_atoms", _I), ("xyz", _DP), ("rms", _DP), ("term_atom", _IP), ("n_terms", _I),
         ("slices", _I), ("slice_origin", _D), ("slice_thickness", _D), ("mid", _IP), ("n_mid", _I),
+        ("multiceng", _D),
     ]
 
 
@@ -466,7 +467,7 @@ class DeviceSpecimen:
     (``stem.specimen_rows``), ``term_atom`` the atom of each row (ele_n, ele_n_i, ele_n_corr order)."""
 
     def __init__(self, paraflag, np_side, rows, term_atom, xyz, rms, kx, ky, s_2, aper_true, sigma, parameter,
-                 slices, slice_origin, slice_thickness, mid=(), corr_info_matrix=None):
+                 slices, slice_origin, slice_thickness, mid=(), corr_info_matrix=None, multiceng=0.0):
         d, keep = _potential_desc(paraflag, np_side, 1, rows["series_n"], rows["ele_n"], kx, ky, s_2, aper_true,
                                   sigma, parameter, atom_slice_i=rows["series_n_i"], proj_coff_mat_i=rows["ele_n_i"],
                                   series_n_corr=rows.get("series_n_corr"), ele_n_corr=rows.get("ele_n_corr"),
@@ -494,6 +495,7 @@ class DeviceSpecimen:
         keep["mid"] = mid
         sd.mid = mid.ctypes.data_as(_IP) if mid.size else None
         sd.n_mid = int(mid.size)
+        sd.multiceng = float(multiceng)
         h = _VP()
         check(lib().totem_specimen_create(ctypes.byref(sd), ctypes.byref(h)))
         self._h = h

@@ -133,7 +133,8 @@ def device_specimen(cfg, sigma_by_z=None, rms=None, atoms=None):
     """The specimen resident on the GPU (:class:`totem_amd.engine.DeviceSpecimen`), displaced and
     sliced there per frozen-phonon configuration.  Displacement rms per atom from ``sigma_by_z``
     ({Z: rms}, Angstrom; the Debye-Waller column is then cleared as in :func:`phonon_snapshots`) or
-    ``rms`` directly; neither gives a static specimen."""
+    ``rms`` directly; neither gives a static specimen.  ``cfg.multiceng`` > 0 (Peng) spreads atoms
+    near faces over neighbouring slices as :func:`prepare` does."""
     from . import engine
     x = np.array(cfg.atoms if atoms is None else atoms, dtype=np.float64)
     if sigma_by_z is not None:
@@ -144,7 +145,7 @@ def device_specimen(cfg, sigma_by_z=None, rms=None, atoms=None):
     return engine.DeviceSpecimen(cfg.paraflag, cfg.np_side, rows, term_atom, x[:, 1:4], rms, gx, gy, s2,
                                  np.ones_like(s2), optics.interaction_sigma(cfg.kv),
                                  optics.potential_parameter(cfg.np_side, cfg.dx), cfg.n_slices, cfg.slice_origin,
-                                 cfg.slice_thickness, tuple(cfg.mid_slices), cim)
+                                 cfg.slice_thickness, tuple(cfg.mid_slices), cim, cfg.multiceng)
 
 
 def prepare(cfg, atoms=None, optics_too=True):
