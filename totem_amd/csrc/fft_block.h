@@ -216,12 +216,30 @@ template <int R, int SIGN> __device__ __forceinline__ void dft(float2* a) {
 // a per-lane base plus compile-time offsets, which the ds_read/ds_write immediate
 // offsets absorb.
 
+// Exchange barrier: the whole workgroup, or only the wave when the layout keeps
+// every transform inside one wave (LDS operations of a wave complete in order,
+// so a compiler-level fence is enough there).
+template <bool WAVE> __device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, int... RS> struct Stages;
 
 // Twiddles of the stage (P, R) for the U butterflies of lane t: block k = b mod P
 // of the stage table, R entries (index u*R + q; entry 0 is 1).  A stage's loads
 // are issued by the previous stage before its LDS exchange, so their latency
 // overlaps the exchange instead of stalling the butterflies after the barrier.
+// TOTEM_TW_LATE issues them after the previous stage's butterflies (its own
+// twiddles are dead by then, so the two sets share registers) instead of before.
+#ifndef TOTEM_TW_LATE
+#define TOTEM_TW_LATE 1
+#endif
 template <int T, int P, int R, int U> struct StageTw {
   static constexpr int NW = (P > 1) ? U * R : 1;
   float2 w[NW];
@@ -270,7 +288,7 @@ struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
     float2* buf = PING ? lds1 : lds0;
     const float2* tw_next = tw + (P > 1 ? P * R : 0);
     typename Next::Tw nxt;
-    if constexpr (!LAST) nxt.load(tw_next, t);
+    if constexpr (!LAST && !TOTEM_TW_LATE) nxt.load(tw_next, t);
     float2 out[E];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -307,11 +325,12 @@ struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
 #pragma unroll
       for (int v = 0; v < E; ++v) x[v] = out[v];
     } else {
-      __syncthreads();
+      if constexpr (TOTEM_TW_LATE) nxt.load(tw_next, t);
+      lds_sync<Lay::kWave>();
       const float2* rp = buf + lay(t);
 #pragma unroll
       for (int v = 0; v < E; ++v) x[v] = rp[Lay::off(T * v)];
-      if constexpr (!PP) __syncthreads();
+      if constexpr (!PP) lds_sync<Lay::kWave>();
       Next::run(x, lds0, lds1, lay, t, tw_next, nxt);
     }
   }
@@ -367,7 +386,8 @@ template <int N, bool PP> constexpr int next_start(int prev) { return PP ? (prev
 // first stage's stride-R writes spread over the banks.  Column transforms:
 // C interleaved columns, element e of column c at e*C + c (conflict-free
 // because the C lanes of a group touch consecutive slots).
-struct RowLay {
+template <bool WAVE> struct RowLayT {
+  static constexpr bool kWave = WAVE;  // each transform's lanes lie in one wave
   int base;
   __device__ __forceinline__ int operator()(int e) const { return base + e + (e >> 4); }
   // slot(e + k) - slot(e) for k a multiple of 16
@@ -377,7 +397,9 @@ struct RowLay {
 // Column transforms also pad one slot-row per 16 elements: without it the first
 // stage's stride-R writes of lanes t, t+1, ... land R*C slots apart, i.e. on the
 // same banks.
+using RowLay = RowLayT<false>;
 template <int C> struct ColLay {
+  static constexpr bool kWave = false;
   int c;
   __device__ __forceinline__ int operator()(int e) const { return (e + (e >> 4)) * C + c; }
   static __device__ __forceinline__ constexpr int off(int k) { return (k + (k >> 4)) * C; }

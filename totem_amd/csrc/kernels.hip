@@ -26,6 +26,9 @@
 #ifndef TOTEM_COLS_C1024
 #define TOTEM_COLS_C1024 4     // columns per workgroup of the 1024-point columns pass
 #endif
+#ifndef TOTEM_COLS_C1536
+#define TOTEM_COLS_C1536 4     // ... and of the 1536- and 2048-point ones
+#endif
 #ifndef TOTEM_ROWS_PP
 #define TOTEM_ROWS_PP 0        // rows pass: ping-pong LDS (1) or one buffer (0)
 #endif
@@ -56,7 +59,7 @@ template <int N> struct ColCfg {
   static constexpr int NBUF = PP ? 2 : 1;
   static constexpr int T = Plan<N>::T;
   // columns per workgroup
-  static constexpr int C = (N <= 512) ? 16 : (N >= 1536 ? 4 : (N == 1024 ? TOTEM_COLS_C1024 : 8));
+  static constexpr int C = (N <= 512) ? 16 : (N >= 1536 ? TOTEM_COLS_C1536 : (N == 1024 ? TOTEM_COLS_C1024 : 8));
   static constexpr int THREADS = T * C;
 };
 
@@ -95,6 +98,21 @@ template <int N> struct WaveLay {
 // the LDS rows are then padded by ROWS_LDS_PAD float2 so that the TR rows of a wave
 // fall on different banks (measured: conflict-free, but no faster at 1024^2).
 constexpr int ROWS_LDS_PAD = TOTEM_ROWS_LANE_TILED ? 4 : 0;
+#ifndef TOTEM_ROWS_WAVESYNC
+#define TOTEM_ROWS_WAVESYNC 1  // rows that fit in one wave exchange with wave-local barriers
+#endif
+#ifndef TOTEM_ROWS_WAVE1024
+#define TOTEM_ROWS_WAVE1024 0  // ... also at 1024 points
+#endif
+// Row-major lanes with T <= 64 keep each row inside one wave: the waves of a
+// rows-pass workgroup then run their transforms independently.  Measured with
+// cache-fit batches (us per launch, workgroup -> wave barriers): 256: 94 -> 88,
+// 512: 103.5 -> 100, 768: 177 -> 100, 1536: 185 -> 116 (the 2-wave/SIMD plans
+// gain most), but 1024: 110.5 -> 113, so 1024 keeps workgroup barriers.
+template <int N> constexpr bool rows_wave_local() {
+  return TOTEM_ROWS_WAVESYNC && !TOTEM_ROWS_LANE_TILED && Plan<N>::T <= 64 && 64 % Plan<N>::T == 0 &&
+         (N != 1024 || TOTEM_ROWS_WAVE1024);
+}
 template <int N> __device__ __forceinline__ void row_lane(int tid, int& lr, int& t) {
   constexpr int T = RowCfg<N>::T, TR = TOTEM_ROWS_LANE_TILED ? WaveLay<N>::TR : 1;
   const int q = tid / (TR * T), rem = tid - q * (TR * T);
@@ -192,7 +210,7 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   float2* const lds1 = lds + (Cfg::NBUF - 1) * RW * LLP;
   int lr, t;
   row_lane<N>(threadIdx.x, lr, t);
-  const RowLay lay{lr * LLP};
+  const RowLayT<rows_wave_local<N>()> lay{lr * LLP};
   constexpr int DSTEP = WaveLay<N>::row_step(T);
   float2 xa[E], xb[E];
   RowUnit ua = row_unit<N, K>(items, jobs, step, 0, lr, t);
