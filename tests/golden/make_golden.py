@@ -17,6 +17,11 @@ parameter structs, U/V grids and exit wave (320^2) go to ``pydata_inputs.npz``; 
 batch HRTEM images for seeded draws (``HRTEM_CASES``) go to ``pydata_golden.npz`` as the crops the
 tool's consumer cuts (``position`` from ``othervar``) plus per-image sums.
 
+A third set, ``allresul_phonon.npz``, holds frozen-phonon displacements of the stored specimen: the
+oracle's positions (``oracle/phonon.py``) for ``PHONON`` (seed, configurations, rms per element,
+mid-slice faces) and the per-slice atom counts of each configuration.  It needs only the stored
+specimen: ``python tests/golden/make_golden.py phonon``.
+
 Only this script reads the reference checkout; the tests read the ``.npz`` files.
 """
 import os
@@ -114,6 +119,32 @@ def hrtem_golden(inputs):
     return out
 
 
+PHONON = dict(seed=20241202, configs=(0, 1, 5), rms_by_z={38: 0.08, 22: 0.08, 8: 0.10}, mid=(10, 25))
+
+
+def phonon_golden():
+    sys.path.insert(0, ROOT)
+    from oracle import phonon as oph
+    spec = np.load(os.path.join(HERE, "allresul_specimen.npz"))
+    atoms = spec["atoms"]
+    faces = spec["slicethick"]
+    dz = float(spec["eachthick"])
+    S = len(faces)
+    rms = np.array([PHONON["rms_by_z"][int(z)] for z in atoms[:, 0]])
+    lo, hi, mid_faces = oph.slab(float(faces[0]), dz, S, PHONON["mid"])
+    out = {"seed": np.uint64(PHONON["seed"]), "configs": np.asarray(PHONON["configs"], dtype=np.int32),
+           "rms": rms, "mid": np.asarray(PHONON["mid"], dtype=np.int32)}
+    pos, counts = [], []
+    for c in PHONON["configs"]:
+        p = oph.displace(atoms[:, 1:4], rms, PHONON["seed"], c, lo, hi, mid_faces)
+        pos.append(p)
+        counts.append([len(m) for m in oph.slice_members(p[:, 2], float(faces[0]), dz, S)])
+    out["positions"] = np.stack(pos)
+    out["slice_counts"] = np.asarray(counts, dtype=np.int32)
+    np.savez_compressed(os.path.join(HERE, "allresul_phonon.npz"), **out)
+    print("phonon", out["positions"].shape, out["slice_counts"].sum(axis=1), flush=True)
+
+
 def main():
     import scipy.io as sio
     m = sio.loadmat(REF_PY)
@@ -145,7 +176,11 @@ def main():
             out["%s_%s" % (name, k)] = v
         print(name, "myresul", res.ravel()[:4], "mid", mid.shape, flush=True)
     np.savez_compressed(os.path.join(HERE, "allresul_golden.npz"), **out)
+    phonon_golden()
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["phonon"]:
+        phonon_golden()
+    else:
+        main()

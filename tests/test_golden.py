@@ -1,10 +1,11 @@
 """Golden fixtures on the reference's own specimen file (``allresul.mat``: 791-atom SrTiO3,
 41 slices of 0.823175 A), written by ``tests/golden/make_golden.py``.
 
-CPU: the oracle and the host set-up reproduce the stored FP64 outputs.
+CPU: the oracle and the host set-up reproduce the stored FP64 outputs, and the oracle's frozen-phonon
+displacements of the specimen (with the host slicer's per-slice counts) the stored ones.
 GPU: the engine, through ``myProjectfun`` and ``totem_build_transmission``, meets the
 stated tolerances against them (potential <= 1e-6 relative L2, detector images
-<= 1e-4 relative per pixel).
+<= 1e-4 relative per pixel), and the device displacements equal the stored ones (<= 1e-12 A).
 """
 import os
 
@@ -74,3 +75,40 @@ def test_engine_matches_golden(engine, name):
     assert err <= 1e-6
     assert np.all(np.abs(n2[live] / ref_n2[live] - 1) <= 2e-6)
     assert np.all(n2[~live] == 0)
+
+
+def _phonon():
+    spec, _ = _load()
+    return spec, dict(np.load(os.path.join(HERE, "allresul_phonon.npz")))
+
+
+def test_oracle_phonon_reproduces_golden():
+    from oracle import phonon as oph
+    from totem_amd import specimen
+    spec, ph = _phonon()
+    atoms, faces, dz = spec["atoms"], spec["slicethick"], float(spec["eachthick"])
+    lo, hi, mid_faces = oph.slab(float(faces[0]), dz, len(faces), tuple(ph["mid"]))
+    for k, c in enumerate(ph["configs"]):
+        p = oph.displace(atoms[:, 1:4], ph["rms"], int(ph["seed"]), int(c), lo, hi, mid_faces)
+        np.testing.assert_array_equal(p, ph["positions"][k])
+        # the host slicer assigns the displaced specimen as the stored counts say
+        x = atoms.copy()
+        x[:, 1:4] = p
+        all_n, all_n_i = specimen.atom_parameters(x, "l")
+        _, _, series, _ = specimen.slice_atoms(all_n, all_n_i, dz, faces, 0, "l", 0.0)
+        np.testing.assert_array_equal(series, ph["slice_counts"][k])
+
+
+@pytest.mark.gpu
+def test_device_phonon_matches_golden(engine):
+    from totem_amd import stem
+    print("case allresul phonon positions", flush=True)
+    spec, ph = _phonon()
+    cfg = mg.config(spec, "lobato")
+    assert tuple(cfg.mid_slices) == tuple(ph["mid"])
+    with stem.device_specimen(cfg, sigma_by_z=mg.PHONON["rms_by_z"]) as sp:
+        for k, c in enumerate(ph["configs"]):
+            p = sp.positions(int(ph["seed"]), int(c))
+            err = float(np.abs(p - ph["positions"][k]).max())
+            print("  config %d: max |device - golden| %.2e A" % (c, err))
+            assert err <= 1e-12

@@ -833,7 +833,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   auto record = [&](const ItemDev* it, int cnt) {
     for (int s = 0; s < S; ++s) {
       pass_rows(n, st, it, cnt, w.job, s);
-      pass_cols(n, st, it, cnt, w.job, s, s == S - 1 || mos[s] >= 0);
+      pass_cols(n, st, it, cnt, w.job, s);
     }
     reduce_det(st, it, nullptr, cnt, w.job, G, L + 1, A);
   };
@@ -852,7 +852,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
       for (int s = 0; s < S; ++s) {
         pass_rows(n, st, ci, cnt, w.job, s);
         HIP_OK(hipEventRecord(ev[2 * s + 1], st));
-        pass_cols(n, st, ci, cnt, w.job, s, s == S - 1 || mos[s] >= 0);
+        pass_cols(n, st, ci, cnt, w.job, s);
         HIP_OK(hipEventRecord(ev[2 * s + 2], st));
       }
       reduce_det(st, ci, nullptr, cnt, w.job, G, L + 1, A);

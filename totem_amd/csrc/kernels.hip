@@ -315,7 +315,7 @@ struct ColUnit {
 
 template <int N, int K>
 __device__ __forceinline__ ColUnit col_unit(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs,
-                                            int step, int u, bool det) {
+                                            int step, int u) {
   using Cfg = ColCfg<N>;
   constexpr int G = N / Cfg::C;
   ColUnit r;
@@ -327,14 +327,17 @@ __device__ __forceinline__ ColUnit col_unit(const ItemDev* __restrict__ items, c
   r.active = r.slice >= 0 && r.slice < r.jb->S;
   r.wr = buf_rsrc(it.wave, (unsigned)N * N * sizeof(float2));
   r.last = r.slice == r.jb->S - 1;
-  r.mid = (det && r.active && r.jb->mid_of_slice) ? r.jb->mid_of_slice[r.slice] : -1;
+  r.mid = (r.active && r.jb->mid_of_slice) ? r.jb->mid_of_slice[r.slice] : -1;
   r.pbase = it.part;
   r.out = it.out;
   return r;
 }
 
-// DET = false: no item of the launch is at a detector slice (the lean variant).
-template <int N, bool DET>
+// One kernel for every slice; the detector work is a run-time branch (uniform
+// per workgroup).  A variant compiled without the detector code was measured
+// slower at every size (its register allocation gives fewer waves per SIMD:
+// at 1024 points 160 vs 128 VGPRs, 101 vs 89 us per launch of 26 probes).
+template <int N>
 __global__ void __launch_bounds__(ColCfg<N>::THREADS, TOTEM_COLS_MINWAVES)
 k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step,
             const float2* __restrict__ tw) {
@@ -343,7 +346,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   constexpr int NW = (Cfg::THREADS + 63) / 64;
   constexpr int LC = col_lds_len<N, C>();
   __shared__ float2 lds[Cfg::NBUF * LC];
-  __shared__ double red[DET ? NW * kMaxAper : 1];
+  __shared__ double red[NW * kMaxAper];
   float2* const lds1 = lds + (Cfg::NBUF - 1) * LC;
   const int c = threadIdx.x % C;
   const int t = threadIdx.x / C;
@@ -351,7 +354,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   constexpr unsigned kBytes = (unsigned)N * N * sizeof(float2);
   constexpr int kStep = T * N * (int)sizeof(float2);
   float2 xa[E], xb[E];
-  ColUnit ua = col_unit<N, K>(items, jobs, step, 0, DET);
+  ColUnit ua = col_unit<N, K>(items, jobs, step, 0);
   // byte offsets of (row t, column): in the wave (WaveLay) and in row-major arrays (H, exit wave);
   // both advance by T*N elements per v
   int voffa = WaveLay<N>::at(t, ua.cg * C + c) * (int)sizeof(float2);
@@ -365,7 +368,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
     ColUnit ub;
     int voffb = 0, hoffb = 0;
     if (u + 1 < K) {
-      ub = col_unit<N, K>(items, jobs, step, u + 1, DET);
+      ub = col_unit<N, K>(items, jobs, step, u + 1);
       voffb = WaveLay<N>::at(t, ub.cg * C + c) * (int)sizeof(float2);
       hoffb = (t * N + ub.cg * C + c) * (int)sizeof(float2);
       if (ub.active) {
@@ -386,7 +389,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
 #pragma unroll
       for (int v = 0; v < E; ++v) xa[v] = cmul(xa[v], TOTEM_COLS_HPREFETCH ? hv[v] : bld(hr, hoffa, v * kStep));
       bool keep = true;
-      if constexpr (DET) {
+      {
         if (ua.mid >= 0) {
           for (int a0 = 0; a0 < jb.A; a0 += kMaxAper)
             detector_partial<N, E, T, C>(xa, jb.aper2 + (size_t)a0 * N * N, false, min(kMaxAper, jb.A - a0), t,
@@ -635,15 +638,12 @@ template <int N> void launch_pass_rows(hipStream_t st, const ItemDev* items, int
   hipLaunchKernelGGL(k_pass_rows<N>, grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, tw_table(N));
 }
 template <int N> void launch_pass_cols(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
-                                       int step, bool det) {
+                                       int step) {
   using Cfg = ColCfg<N>;
   constexpr int K = TOTEM_COLS_K;
   static_assert((N / Cfg::C) % K == 0, "column groups per probe must be a multiple of K");
   const dim3 grid((unsigned)(n_items * (N / Cfg::C) / K));
-  if (det)
-    hipLaunchKernelGGL((k_pass_cols<N, true>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, tw_table(N));
-  else
-    hipLaunchKernelGGL((k_pass_cols<N, false>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, tw_table(N));
+  hipLaunchKernelGGL((k_pass_cols<N>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, tw_table(N));
 }
 template <int N> void launch_fft2d(hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi,
                                    float sigma) {
@@ -704,8 +704,8 @@ void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const J
 #undef X
 }
 
-void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step, bool det) {
-#define X(S) if (n == S) { launch_pass_cols<S>(st, items, n_items, jobs, step, det); return; }
+void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step) {
+#define X(S) if (n == S) { launch_pass_cols<S>(st, items, n_items, jobs, step); return; }
   TOTEM_FOR_SIZES(X)
 #undef X
 }

@@ -23,7 +23,7 @@ int detector_groups(int n);   // column groups per probe in k_pass_cols
 void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
 // det: some item of the launch may be at a detector slice (mid-slice or exit face);
 // false selects the variant without detector code.
-void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step, bool det);
+void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
 // Detector sums of the items items[list[e]] (list may be null: items[e]), e < n, into their
 // job's det; lmax1 and amax bound (L + 1) and A over the jobs involved.
 void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs, int groups,

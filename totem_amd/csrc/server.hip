@@ -230,14 +230,8 @@ void step_group(Server& s, Group& g) {
   const int cnt = (int)g.host_items.size();
   const ItemDev* items = g.items_dev.as<ItemDev>();
   const JobDev* jobs = s.jobs_dev.as<JobDev>();
-  bool det = false;
-  for (const Cohort& c : g.cohorts) {
-    const Job& J = s.jobs.at(c.job);
-    const int slice = g.step - c.start;
-    det = det || slice == J.j.slices - 1 || J.mos[slice] >= 0;
-  }
   pass_rows(g.n, g.st, items, cnt, jobs, g.step);
-  pass_cols(g.n, g.st, items, cnt, jobs, g.step, det);
+  pass_cols(g.n, g.st, items, cnt, jobs, g.step);
   bool removed = false;
   for (Cohort& c : g.cohorts) {
     Job& J = s.jobs.at(c.job);
