@@ -73,11 +73,10 @@ def measured_traffic(kname, n, workload):
     path = os.path.join(ROOT, "profiles", "stem_passes_%d_%s_traffic.txt" % (n, workload))
     if not os.path.exists(path):
         return None
-    row = kname.replace(">", ", false>") if kname.startswith("k_pass_cols") else kname
     with open(path) as fh:
         for line in fh:
-            if line.startswith(row + " "):
-                f = line[len(row):].split()
+            if line.startswith(kname + " "):
+                f = line[len(kname):].split()
                 return (float(f[1]) + float(f[3])) * 2**20
     return None
 
