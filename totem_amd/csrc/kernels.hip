@@ -87,34 +87,35 @@ template <int N> struct WaveLay {
   static constexpr int row_step(int T) { return TOTEM_TILED ? 16 * (T / TC) : T; }
 };
 
-#ifndef TOTEM_ROWS_LANE_TILED
-#define TOTEM_ROWS_LANE_TILED 0
-#endif
-// Rows-pass lane -> (row of the workgroup, lane within the row).  Default: T
-// consecutive lanes per row, so the LDS exchanges of a wave stay within one row
-// (conflict-free); the TR rows of a tile are read by TR waves of the workgroup at
-// the same time and share each line in the caches.  With TOTEM_ROWS_LANE_TILED the
-// 64 lanes of a wave cover TR rows x 64/TR columns (whole tiles per instruction);
-// the LDS rows are then padded by ROWS_LDS_PAD float2 so that the TR rows of a wave
-// fall on different banks (measured: conflict-free, but no faster at 1024^2).
-constexpr int ROWS_LDS_PAD = TOTEM_ROWS_LANE_TILED ? 4 : 0;
 #ifndef TOTEM_ROWS_WAVESYNC
 #define TOTEM_ROWS_WAVESYNC 1  // rows that fit in one wave exchange with wave-local barriers
 #endif
-#ifndef TOTEM_ROWS_WAVE1024
-#define TOTEM_ROWS_WAVE1024 0  // ... also at 1024 points
+#ifndef TOTEM_ROWS_LANE_TILED
+#define TOTEM_ROWS_LANE_TILED 2  // 0: never, 1: always, 2: the sizes that keep workgroup barriers
 #endif
-// Row-major lanes with T <= 64 keep each row inside one wave: the waves of a
-// rows-pass workgroup then run their transforms independently.  Measured with
-// cache-fit batches (us per launch, workgroup -> wave barriers): 256: 94 -> 88,
-// 512: 103.5 -> 100, 768: 177 -> 100, 1536: 185 -> 116 (the 2-wave/SIMD plans
-// gain most), but 1024: 110.5 -> 113, so 1024 keeps workgroup barriers.
+// Rows-pass lane mapping, two forms:
+//  * row-major lanes: T consecutive lanes per row, so a row with T <= 64 stays
+//    inside one wave and its LDS exchanges need only wave-local barriers; the TR
+//    rows of a tile are read by TR waves of the workgroup and share each line in
+//    the caches;
+//  * lane-tiled: the 64 lanes of a wave cover TR rows x 64/TR columns, whole
+//    128-byte tiles per load and store instruction; a row then spans TR waves
+//    (workgroup barriers), and the LDS rows are padded by 4 float2 so that the
+//    TR rows of a wave fall on different banks.
+// Measured with cache-fit batches (us per launch of the rows pass, workgroup
+// barriers -> wave-local row-major -> lane-tiled): 256: 94 -> 88 -> 95,
+// 512: 103.5 -> 100 -> 104, 768: 177 -> 100 -> 133, 1024: 110.5 -> 113 -> 107,
+// 1536: 185 -> 116 -> 137, 2048 (T = 128, no wave-local form): 111 -> - -> 97.
+template <int N> constexpr bool rows_lane_tiled() {
+  return TOTEM_ROWS_LANE_TILED == 1 ||
+         (TOTEM_ROWS_LANE_TILED == 2 && !(TOTEM_ROWS_WAVESYNC && Plan<N>::T <= 64 && N != 1024));
+}
+template <int N> constexpr int rows_lds_pad() { return rows_lane_tiled<N>() ? 4 : 0; }
 template <int N> constexpr bool rows_wave_local() {
-  return TOTEM_ROWS_WAVESYNC && !TOTEM_ROWS_LANE_TILED && Plan<N>::T <= 64 && 64 % Plan<N>::T == 0 &&
-         (N != 1024 || TOTEM_ROWS_WAVE1024);
+  return TOTEM_ROWS_WAVESYNC && !rows_lane_tiled<N>() && Plan<N>::T <= 64 && 64 % Plan<N>::T == 0;
 }
 template <int N> __device__ __forceinline__ void row_lane(int tid, int& lr, int& t) {
-  constexpr int T = RowCfg<N>::T, TR = TOTEM_ROWS_LANE_TILED ? WaveLay<N>::TR : 1;
+  constexpr int T = RowCfg<N>::T, TR = rows_lane_tiled<N>() ? WaveLay<N>::TR : 1;
   const int q = tid / (TR * T), rem = tid - q * (TR * T);
   lr = q * TR + rem % TR;
   t = rem / TR;
@@ -205,7 +206,7 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS, TOTEM_ROWS_MINWAVES)
 k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step, const float2* __restrict__ tw) {
   using Cfg = RowCfg<N>;
   constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, K = TOTEM_ROWS_K;
-  constexpr int LLP = Cfg::LL + ROWS_LDS_PAD;
+  constexpr int LLP = Cfg::LL + rows_lds_pad<N>();
   __shared__ float2 lds[Cfg::NBUF * RW * LLP];
   float2* const lds1 = lds + (Cfg::NBUF - 1) * RW * LLP;
   int lr, t;
