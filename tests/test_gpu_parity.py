@@ -175,6 +175,19 @@ def test_stem_myprojectfun_matches_oracle(engine, flag, M, mid):
     assert np.all(px == 0) and np.all(py == 0)
 
 
+@pytest.mark.parametrize("flag,n,np_side,S", [("n", 320, 512, 4), ("n", 768, 1024, 4), ("l", 1536, 2048, 2)])
+def test_stem_mixed_radix_grids_match_oracle(engine, flag, n, np_side, S):
+    """Windows of 320 (radix 5), 768 and 1536 (radix 3) points -- the plans whose rows pass exchanges
+    with wave-local barriers -- inside a larger potential, with a mid-slice output."""
+    print("case stem mixed-radix n=%d np=%d S=%d flag=%s" % (n, np_side, S, flag), flush=True)
+    cfg, inp = _stem_case(flag, M=1, mid=(1,), n=n, np_side=np_side, S=S, cells=2, scan=(2, 1), step=7,
+                          begin=(11, 23))
+    myresul, mid_mat, _, _ = stem.run(inp)
+    res_o, mid_o = _oracle_stem(inp)
+    _check_images(myresul, res_o, "exit-face detectors")
+    _check_images(mid_mat, mid_o, "mid-slice detectors")
+
+
 def test_stem_window_outside_potential_is_an_error(engine):
     cfg, inp = _stem_case("l", scan=(3, 2), step=60, begin=(20, 30))
     with pytest.raises(engine.TotemError, match="window leaves the potential"):
