@@ -81,6 +81,43 @@ def measured_traffic(kname, n, workload):
     return None
 
 
+def run_hrtem_batch(n_images, cpu_images, seed, n=320):
+    """The batch HRTEM tool's producer step (cuda_stem64.py:164-704 call shape: draw aberrations,
+    transfer function x exit wave, inverse FFT, |.|^2) on synthetic inputs of the tool's shape: a
+    320^2 exit wave, 80 kV, every aberration drawn.  Returns images/s through the host API (host
+    exit wave in, host images out, as the tool's queue receives them) and the FP64 oracle's rate on
+    the host for a few of the same images, with its error."""
+    from oracle import hrtem as ohr
+    from totem_amd import hrtem
+    rng = np.random.default_rng(seed)
+    lam, sampling = 0.00417552, 0.01
+    f = (np.arange(n) - n // 2) / (n * sampling)
+    U, V = np.meshgrid(f, f)
+    w = np.exp(1j * rng.uniform(-0.5, 0.5, (n, n)))
+    mywave = np.fft.fftshift(np.fft.fft2(w))
+    pp2 = np.zeros(49)
+    pp2[0], pp2[1], pp2[2], pp2[8] = lam, 10.0, 10.0, 1.0e3
+    mrr = np.abs(rng.normal(0.0, 5.0, 21)) + 1.0
+    pp1 = {"lambda": lam, "alafa": 3e-4, "delta_yita": 2.0, "gfsf": 1.0, "mtotal": 1}
+    ai = rng.random((n_images, 21))
+    hrtem.images(pp1, pp2, mrr, U, V, mywave, ai[:8])          # warm-up (plans, buffers)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        img, paras = hrtem.images(pp1, pp2, mrr, U, V, mywave, ai)
+    gpu_wall = (time.perf_counter() - t0) / reps
+    k = min(cpu_images, n_images)
+    t1 = time.perf_counter()
+    ref = ohr.images(mywave, paras[:k], U, V, lam, pp1["alafa"], 1, pp1["delta_yita"], pp1["gfsf"])
+    cpu_wall = time.perf_counter() - t1
+    err = max(float(np.linalg.norm(img[b] - ref[b]) / np.linalg.norm(ref[b])) for b in range(k))
+    return {"grid": n, "images_per_call": n_images, "images_per_s": n_images / gpu_wall,
+            "ms_per_call": gpu_wall * 1e3, "includes": "host->device inputs, all kernels, images back to host",
+            "cpu_oracle": {"value": k / cpu_wall, "unit": "images/s", "cores": 1, "kind": "port",
+                           "sample": "%d images, FP64 numpy oracle (oracle/hrtem.py)" % k},
+            "oracle_max_rel_l2": err, "seed": seed}
+
+
 def run_serving(n_requests, rate, seed, arena_gib, budget_gib):
     """Poisson arrivals of mixed-size STEM requests (grids 256^2 50%, 512^2 30%, 1024^2 15%,
     2048^2 5%; 10-120 slices; scans 8x8-64x64; Lobato or Peng; SrTiO3 or amorphous C) through the
@@ -163,6 +200,9 @@ def main():
     ap.add_argument("--uneven-jobs", type=int, default=48, help="jobs of the uneven-batch leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the mixed serving load (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--hrtem-images", type=int, default=256,
+                    help="images per call of the batch HRTEM leg (0: skip)")
+    ap.add_argument("--hrtem-only", action="store_true", help="run only the batch HRTEM leg (profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-traffic kernel for "
                          "scaling hardware byte counters)")
@@ -177,6 +217,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     engine.lib()
+    if args.hrtem_only:
+        print(json.dumps({"hrtem_batch": run_hrtem_batch(max(args.hrtem_images, 1), 4, seed=1006)}), flush=True)
+        return
     dev = torch.device("cuda:0")
     if args.workload == "c2":
         cfg = stem.perovskite_c2(scan=args.scan)
@@ -336,11 +379,13 @@ def main():
                                          "oracle (propagation only, transmission taken from the GPU build), "
                                          "%.1f s wall" % (len(positions), S, n_cfg - 1, cwall)}
         out["oracle_max_rel_err"] = rel
-    if args.uneven_jobs > 0 or args.serve_requests > 0:
+    if args.uneven_jobs > 0 or args.serve_requests > 0 or args.hrtem_images > 0:
         if not args.host_slicing:
             spec.close()
         del ws, t, probes, det
         torch.cuda.empty_cache()
+    if args.hrtem_images > 0:
+        out["hrtem_batch"] = run_hrtem_batch(args.hrtem_images, 8, seed=1006)
     if args.uneven_jobs > 0:
         out["uneven_batch"] = run_uneven(args.uneven_jobs, seed=1004, arena_gib=0.25)
     if args.serve_requests > 0:

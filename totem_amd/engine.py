@@ -341,10 +341,17 @@ def hrtem_images(n, lambda_nm, alafa, delta_yita, gfsf, mtotal, U, V, mywave, pp
     mrr = np.ascontiguousarray(keep[4][:21])
     keep.append(mrr)
     d.mrr = _dp(mrr)
-    images = np.zeros((B, n, n))
+    images = _pinned_host((B, n, n))
     paras = np.zeros((B, 49))
     check(lib().totem_hrtem_images(ctypes.byref(d), _dp(paras), _dp(images), None))
     return images, paras
+
+
+def _pinned_host(shape):
+    """A float64 host array in page-locked memory (torch's caching host allocator), so that large
+    device-to-host copies run at the link rate instead of through pageable staging."""
+    import torch
+    return torch.empty(shape, dtype=torch.float64, pin_memory=True).numpy()
 
 
 def tcc_image(wave, tcc, ap, gfsf):
