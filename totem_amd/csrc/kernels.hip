@@ -619,6 +619,85 @@ k_accumulate(const float2* __restrict__ ex, const float2* __restrict__ ey, const
   }
 }
 
+// The same sum as a complex matrix product on the matrix cores: per class,
+// A[r][j] = ey_j[r] (32 rows x 2 terms) times B[j][c] = w_j ex_j[c] (2 terms x 32
+// columns) accumulates a 32 x 32 pixel tile in FP32 with four real
+// v_mfma_f32_32x32x2f32 per pair of terms (Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br);
+// the tile is then weighted by the class's FP64 form factor per pixel.  One wave
+// per tile.  Operand layout of the 32x32x2 instruction: lane l supplies
+// A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator element i of lane l is
+// C[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
+#ifndef TOTEM_ACC_MFMA
+#define TOTEM_ACC_MFMA 1
+#endif
+constexpr int kMfmaTile = 32;
+#ifndef TOTEM_ACC_PAIRS
+#define TOTEM_ACC_PAIRS 4
+#endif
+constexpr int kMfmaPairs = TOTEM_ACC_PAIRS;
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__global__ void __launch_bounds__(64)
+k_accumulate_mfma(const float2* __restrict__ ex, const float2* __restrict__ ey, const double* __restrict__ wt,
+                  const PotSeg* __restrict__ segs, int n_segs, const PotClass* __restrict__ classes,
+                  const double* __restrict__ s2, const double* __restrict__ aper_true,
+                  const double* __restrict__ corr, double parameter, int np, float2* __restrict__ vshift) {
+  const int lane = threadIdx.x;
+  const int tiles = np / kMfmaTile;
+  const int tr = blockIdx.x / tiles, tc = blockIdx.x - tr * tiles;
+  const int r0 = tr * kMfmaTile, c0 = tc * kMfmaTile;
+  const int li = lane & 31, lk = lane >> 5;
+  const size_t np2 = (size_t)np * np;
+  double vr[16], vi[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) vr[i] = vi[i] = 0.0;
+  for (int sgi = 0; sgi < n_segs; ++sgi) {
+    const PotSeg sg = segs[sgi];
+    floatx16 cr = {}, ci = {};
+    // kMfmaPairs pairs of terms per step: all their loads are issued before the
+    // first product, so one wait covers the latency of the whole group
+    for (int j = sg.begin; j < sg.end; j += 2 * kMfmaPairs) {
+      float2 a[kMfmaPairs], b[kMfmaPairs];
+#pragma unroll
+      for (int u = 0; u < kMfmaPairs; ++u) {
+        const int jj = j + 2 * u + lk;
+        a[u] = b[u] = make_float2(0.f, 0.f);
+        if (jj < sg.end) {
+          a[u] = ey[(size_t)jj * np + r0 + li];
+          const float w = (float)wt[jj];
+          const float2 e = ex[(size_t)jj * np + c0 + li];
+          b[u] = make_float2(w * e.x, w * e.y);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kMfmaPairs; ++u) {
+        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u].x, b[u].x, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u].x, b[u].y, ci, 0, 0, 0);
+        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a[u].y, b[u].y, cr, 0, 0, 0);
+        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u].y, b[u].x, ci, 0, 0, 0);
+      }
+    }
+    const PotClass& cls = classes[sg.cls];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int r = r0 + 8 * (i / 4) + 4 * lk + (i % 4), c = c0 + li;
+      const size_t pix = (size_t)r * np + c;
+      const double2 f = form_factor(cls, s2[pix], corr, pix, np2);
+      vr[i] += f.x * cr[i] - f.y * ci[i];
+      vi[i] += f.x * ci[i] + f.y * cr[i];
+    }
+  }
+  const int half = np / 2;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = r0 + 8 * (i / 4) + 4 * lk + (i % 4), c = c0 + li;
+    const size_t pix = (size_t)r * np + c;
+    const double scale = parameter * aper_true[pix];
+    const int rs = (r + half) % np, cs = (c + half) % np;
+    vshift[(size_t)rs * np + cs] = make_float2((float)(vr[i] * scale), (float)(vi[i] * scale));
+  }
+}
+
 __global__ void k_fill_c64(float2* __restrict__ p, long n, float re, float im) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = make_float2(re, im);
@@ -742,6 +821,12 @@ void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_term
 void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,
                 const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
                 double parameter, int np, float2* vshift) {
+  if (TOTEM_ACC_MFMA && np % kMfmaTile == 0) {
+    const int tiles = np / kMfmaTile;
+    hipLaunchKernelGGL(k_accumulate_mfma, dim3((unsigned)(tiles * tiles)), dim3(64), 0, st, ex, ey, wt, segs,
+                       n_segs, classes, s2, aper_true, corr, parameter, np, vshift);
+    return;
+  }
   const dim3 grid((np + kAccTile - 1) / kAccTile, (np + kAccTile - 1) / kAccTile);
   hipLaunchKernelGGL(k_accumulate, grid, dim3(256), 0, st, ex, ey, wt, segs, n_segs, classes, s2, aper_true, corr,
                      parameter, np, vshift);
