@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -124,8 +125,11 @@ static void select_device() {
 
 // -------------------------------------------------------------- potential build
 
+// A form-factor class: kind, absorption flag, correction-table indices, weights,
+// B and the three coefficient sets, as 37 doubles compared lexicographically.
+using ClassKeyV = std::array<double, 37>;
 struct ClassKey {
-  std::vector<double> v;
+  ClassKeyV v;
   bool operator<(const ClassKey& o) const { return v < o.v; }
 };
 
@@ -134,7 +138,7 @@ struct Term {
   int cls;
 };
 
-static PotClass make_class(const std::vector<double>& key) {
+static PotClass make_class(const ClassKeyV& key) {
   PotClass k;
   std::memset(&k, 0, sizeof(k));
   k.kind = (int)key[0];
@@ -152,8 +156,8 @@ static PotClass make_class(const std::vector<double>& key) {
   return k;
 }
 
-static std::vector<double> blank_key(int kind) {
-  std::vector<double> k(37, 0.0);
+static ClassKeyV blank_key(int kind) {
+  ClassKeyV k{};
   k[0] = kind;
   return k;
 }
@@ -199,13 +203,23 @@ static PotentialPlan plan_potential(const totem_potential_desc& d) {
   const bool lob = d.paraflag == 'l';
   const int ncol = lob ? 14 : 13, ncol_i = lob ? 26 : 25;
   std::map<ClassKey, int> cmap;
-  auto class_of = [&](std::vector<double>&& key) {
-    ClassKey k{std::move(key)};
+  // rows of one element arrive in runs, so the previous row's class is checked first
+  ClassKeyV last{};
+  int last_id = -1;
+  auto class_of = [&](const ClassKeyV& key) {
+    if (last_id >= 0 && key == last) return last_id;
+    ClassKey k{key};
     auto it = cmap.find(k);
-    if (it != cmap.end()) return it->second;
-    const int id = (int)P.classes.size();
-    P.classes.push_back(make_class(k.v));
-    cmap.emplace(std::move(k), id);
+    int id;
+    if (it != cmap.end()) {
+      id = it->second;
+    } else {
+      id = (int)P.classes.size();
+      P.classes.push_back(make_class(k.v));
+      cmap.emplace(k, id);
+    }
+    last = key;
+    last_id = id;
     return id;
   };
   P.slices.assign(S, {});
@@ -223,7 +237,7 @@ static PotentialPlan plan_potential(const totem_potential_desc& d) {
         key[1] = 1;
         for (int q = 0; q < 10; ++q) key[27 + q] = d.absorp_n[a * 10 + q];
       }
-      P.slices[s].push_back({r[0], r[1], r[2], class_of(std::move(key))});
+      P.slices[s].push_back({r[0], r[1], r[2], class_of(key)});
     }
   // atoms with mixed atomic + ionic form factors
   counts_to_offsets(d.atom_slice_i, S, off, "atom_slice_i");
@@ -241,7 +255,7 @@ static PotentialPlan plan_potential(const totem_potential_desc& d) {
         key[1] = 1;
         for (int q = 0; q < 10; ++q) key[27 + q] = d.absorp_n_i[a * 10 + q];
       }
-      P.slices[s].push_back({r[0], r[1], r[2], class_of(std::move(key))});
+      P.slices[s].push_back({r[0], r[1], r[2], class_of(key)});
     }
   // correction terms (Peng "extended"), p_corr: cuda_source_peng_and_corr.cu:652-783
   if (d.cim_len != 0) {
@@ -258,7 +272,7 @@ static PotentialPlan plan_potential(const totem_potential_desc& d) {
         auto key = blank_key(kCorr);
         key[2] = check_cls(r[4]);
         key[6] = r[2];
-        P.slices[s].push_back({r[0], r[1], r[3], class_of(std::move(key))});
+        P.slices[s].push_back({r[0], r[1], r[3], class_of(key)});
       }
     counts_to_offsets(d.series_n_i_corr, S, off, "series_n_i_corr");
     if (off[S] > 0) require(d.ele_n_i_corr != nullptr, "ele_n_i_corr missing");
@@ -271,7 +285,7 @@ static PotentialPlan plan_potential(const totem_potential_desc& d) {
         key[2] = check_cls(r[5]);
         key[5] = r[6];
         key[3] = check_cls(r[7]);
-        P.slices[s].push_back({r[0], r[1], r[3], class_of(std::move(key))});
+        P.slices[s].push_back({r[0], r[1], r[3], class_of(key)});
       }
   }
   // Class ids in key order: any subset of the same classes keeps the same
