@@ -118,6 +118,54 @@ def run_hrtem_batch(n_images, cpu_images, seed, n=320):
             "oracle_max_rel_l2": err, "seed": seed}
 
 
+def run_precision_stress(scan):
+    """The precision-stress specimen (diamond-cubic Si [110] ~100 x 100 x 300 A with 50 A amorphous-C
+    caps, 270 000 atoms) at 2048^2 and ~210 slices of 1.92 A: transmission built on the device from
+    the resident specimen, then a scan x scan probe grid (periodic windows) through every slice."""
+    import torch
+    from totem_amd import engine, optics, stem
+    cfg = stem.precision_c3(scan=scan)
+    S, N, NP = cfg.n_slices, cfg.n, cfg.np_side
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream().cuda_stream
+    t = torch.empty((S, NP, NP), dtype=torch.complex64, device=dev)
+    sp = stem.device_specimen(cfg)
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    sp.transmission(0, 0, t.data_ptr(), stream=st)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - tb
+    sp.close()
+    probe, gfsf = optics.probe_stack(N, cfg.dx, cfg.kv, cfg.alpha_mrad, cfg.aberrations, cfg.focal_members,
+                                     cfg.focal_rms_nm, cfg.source_mrad, cfg.tilt_mrad, cfg.tilt_phi_deg)
+    psf = optics.propagator(N, cfg.dx, cfg.kv, cfg.slice_thickness)
+    aper = optics.detector_masks(N, cfg.dx, cfg.kv, cfg.detectors_mrad)
+    probes = torch.tensor(probe.astype(np.complex64), device=dev)
+    prop = torch.tensor(optics.unit_complex64(psf), device=dev)
+    aper_d = torch.tensor(aper, device=dev)
+    aper2 = torch.tensor(np.fft.ifftshift(aper, axes=(-2, -1)).copy(), device=dev)
+    M, A, D = len(gfsf), aper.shape[0], cfg.depth
+    det = torch.zeros((1, A, M * D), dtype=torch.float64, device=dev)
+    job = engine.stem_job(N, NP, S, t.data_ptr(), probes.data_ptr(), M, prop.data_ptr(), aper_d.data_ptr(), A, D,
+                          cfg.width_red, cfg.step, cfg.begin_row - 1, cfg.begin_col - 1, det.data_ptr(),
+                          aper2_ptr=aper2.data_ptr(), wrap=True)
+    batch = min(M * D, engine.preferred_batch(N))
+    ws_bytes = engine.stem_workspace_bytes(job, batch)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    engine.stem_run(job, batch, workspace_ptr=ws.data_ptr(), workspace_bytes=ws_bytes, stream=st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"workload": "Si [110] ~100x100x300 A + 2x50 A amorphous C, %d atoms, Lobato, 200 kV 22 mrad, "
+                       "ADF 60-200 mrad" % len(cfg.atoms),
+           "grid": N, "slices": S, "probes": M * D, "probes_per_launch": batch,
+           "probe_slices_per_s": M * D * S / wall, "wall_s": wall, "transmission_build_s": build_s}
+    del t, ws, probes, det
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_serving(n_requests, rate, seed, arena_gib, budget_gib):
     """Poisson arrivals of mixed-size STEM requests (grids 256^2 50%, 512^2 30%, 1024^2 15%,
     2048^2 5%; 10-120 slices; scans 8x8-64x64; Lobato or Peng; SrTiO3 or amorphous C) through the
@@ -200,6 +248,8 @@ def main():
     ap.add_argument("--uneven-jobs", type=int, default=48, help="jobs of the uneven-batch leg (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the mixed serving load (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--stress-scan", type=int, default=32,
+                    help="scan side of the 2048^2 precision-stress leg (0: skip)")
     ap.add_argument("--hrtem-images", type=int, default=256,
                     help="images per call of the batch HRTEM leg (0: skip)")
     ap.add_argument("--hrtem-only", action="store_true", help="run only the batch HRTEM leg (profiling)")
@@ -379,13 +429,15 @@ def main():
                                          "oracle (propagation only, transmission taken from the GPU build), "
                                          "%.1f s wall" % (len(positions), S, n_cfg - 1, cwall)}
         out["oracle_max_rel_err"] = rel
-    if args.uneven_jobs > 0 or args.serve_requests > 0 or args.hrtem_images > 0:
+    if args.uneven_jobs > 0 or args.serve_requests > 0 or args.hrtem_images > 0 or args.stress_scan > 0:
         if not args.host_slicing:
             spec.close()
         del ws, t, probes, det
         torch.cuda.empty_cache()
     if args.hrtem_images > 0:
         out["hrtem_batch"] = run_hrtem_batch(args.hrtem_images, 8, seed=1006)
+    if args.stress_scan > 0:
+        out["precision_stress"] = run_precision_stress(args.stress_scan)
     if args.uneven_jobs > 0:
         out["uneven_batch"] = run_uneven(args.uneven_jobs, seed=1004, arena_gib=0.25)
     if args.serve_requests > 0:
