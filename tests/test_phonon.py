@@ -302,3 +302,44 @@ def test_device_erf_weighted_slicing_matches_oracle(engine, flag, multiceng):
               % (label, n_rows, len(xd), len(inp.ele_n_corr) if flag == "n" else "-", e_o, e_h))
         assert n_rows > len(xd)
         assert e_o <= 1e-6 and e_h <= 1e-6
+
+
+def _small_desc_args(flag="n"):
+    x = _ionic_perovskite(4)
+    cfg = stem.StemConfig(atoms=x, np_side=64, dx=0.12, n=32, kv=200.0, alpha_mrad=20.0, slice_thickness=1.95,
+                          n_slices=4, slice_origin=-0.5, paraflag=flag, mid_slices=(2,))
+    rows, ta, cim = stem.specimen_rows(cfg)
+    from totem_amd import optics
+    gx, gy, s2 = optics.reciprocal_grids(cfg.np_side, cfg.dx)
+    return dict(paraflag=flag, np_side=64, rows=rows, term_atom=ta, xyz=x[:, 1:4], rms=np.full(len(x), 0.1), kx=gx,
+                ky=gy, s_2=s2, aper_true=np.ones_like(s2), sigma=0.7, parameter=1.0, slices=4, slice_origin=-0.5,
+                slice_thickness=1.95, mid=(2,), corr_info_matrix=cim)
+
+
+@pytest.mark.parametrize("change,match", [
+    (dict(slices=0), "slices must be positive"),
+    (dict(slice_thickness=0.0), "slice_thickness must be positive"),
+    (dict(multiceng=-1.0), "multiceng must be finite"),
+    (dict(mid=(5,)), "mid slice out of range"),
+    (dict(rms="negative"), "rms must be finite"),
+    (dict(term_atom="short"), "n_terms"),
+    (dict(term_atom="swapped"), "does not sit at its atom"),
+])
+def test_device_specimen_rejects_bad_descriptions(change, match):
+    """Argument checks run before any device call, so they are testable without a GPU."""
+    from totem_amd import engine
+    kw = _small_desc_args()
+    for k, v in change.items():
+        if v == "negative":
+            kw["rms"] = kw["rms"].copy()
+            kw["rms"][3] = -0.1
+        elif v == "short":
+            kw["term_atom"] = kw["term_atom"][:-1]
+        elif v == "swapped":
+            ta = kw["term_atom"].copy()
+            ta[0], ta[1] = ta[1], ta[0]
+            kw["term_atom"] = ta
+        else:
+            kw[k] = v
+    with pytest.raises(engine.TotemError, match=match):
+        engine.DeviceSpecimen(**kw)
