@@ -46,14 +46,18 @@ namespace totem {
 #ifndef TOTEM_ROWS_THREADS
 #define TOTEM_ROWS_THREADS 256  // threads per rows-pass workgroup
 #endif
-#ifndef TOTEM_ROWS_THREADS_1024
-#define TOTEM_ROWS_THREADS_1024 128  // ... at 1024 points
+#ifndef TOTEM_ROWS_THREADS_SMALL
+#define TOTEM_ROWS_THREADS_SMALL 128  // ... at 768 and 1024 points
 #endif
 // Rows per rows-pass workgroup also sets the wave tile (TR = min(RW, 4) rows of
 // 16 / TR columns).  At 1024 points two rows per workgroup (2 x 8 tiles) measured
 // best over the two passes together (rows 110 -> 97 us, columns 90 -> 98 us per
-// launch of 26 probes); 2048 points keep 2 x 8 tiles from 128 lanes per row.
-template <int N> constexpr int rows_threads() { return N == 1024 ? TOTEM_ROWS_THREADS_1024 : TOTEM_ROWS_THREADS; }
+// launch of 26 probes); at 768 points 128-thread workgroups (4 rows, tiles
+// unchanged) take the rows pass from 100.5 to 96.5 us; 2048 points keep 2 x 8
+// tiles from 128 lanes per row.
+template <int N> constexpr int rows_threads() {
+  return (N == 1024 || N == 768) ? TOTEM_ROWS_THREADS_SMALL : TOTEM_ROWS_THREADS;
+}
 template <int N> struct RowCfg {
   static constexpr int T = Plan<N>::T;
   static constexpr int RW = (rows_threads<N>() / T) > 0 ? rows_threads<N>() / T : 1;  // rows per workgroup
