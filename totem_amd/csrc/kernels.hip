@@ -4,7 +4,9 @@
 //  * k_pass_cols  -- per slice, along columns: forward column FFT -> x propagator ->
 //                    [annular detectors] -> inverse column FFT   (exit slice: detectors only)
 //  * k_reduce_det -- per finished probe: fixed-order compensated FP64 sum of detector partials
-//  * k_phase / k_accumulate -- projected potential in reciprocal space per slice
+//  * k_phase / k_accumulate_mfma -- projected potential in reciprocal space per slice
+//                    (structure-factor sums on the matrix cores; k_accumulate is the
+//                    VALU form of the same sum)
 //  * k_fft_rows / k_fft_cols -- plain batched 2-D FFT passes (potential IFFT, tests)
 //
 // A full 2-D forward+inverse FFT per slice therefore costs two read+write
