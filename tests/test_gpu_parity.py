@@ -188,6 +188,22 @@ def test_stem_mixed_radix_grids_match_oracle(engine, flag, n, np_side, S):
     _check_images(mid_mat, mid_o, "mid-slice detectors")
 
 
+def test_empty_slices_are_vacuum(engine):
+    """Slices without atoms (here the last four of six) transmit exactly: t = 1 + 0i, and the scan
+    images still match the oracle."""
+    print("case empty slices", flush=True)
+    cfg, inp = _stem_case("l", S=6, scan=(2, 2), step=5)
+    keep = cfg.atoms[:, 3] < cfg.slice_origin + 2 * cfg.slice_thickness
+    cfg.atoms = cfg.atoms[keep]
+    inp = stem.prepare(cfg)
+    assert np.count_nonzero(inp.series_n[2:]) == 0 and np.count_nonzero(inp.series_n[:2]) == 2
+    _, t = _build_gpu(engine, inp)
+    assert np.all(t[2:] == 1.0)
+    myresul, _, _, _ = stem.run(inp)
+    res_o, _ = _oracle_stem(inp)
+    _check_images(myresul, res_o, "exit-face detectors")
+
+
 def test_stem_window_outside_potential_is_an_error(engine):
     cfg, inp = _stem_case("l", scan=(3, 2), step=60, begin=(20, 30))
     with pytest.raises(engine.TotemError, match="window leaves the potential"):
