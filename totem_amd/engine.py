@@ -349,8 +349,12 @@ def hrtem_images(n, lambda_nm, alafa, delta_yita, gfsf, mtotal, U, V, mywave, pp
 
 def _pinned_host(shape):
     """A float64 host array in page-locked memory (torch's caching host allocator), so that large
-    device-to-host copies run at the link rate instead of through pageable staging."""
-    import torch
+    device-to-host copies run at the link rate instead of through pageable staging; ordinary
+    pageable memory when torch is not importable."""
+    try:
+        import torch
+    except ImportError:
+        return np.zeros(shape)
     return torch.empty(shape, dtype=torch.float64, pin_memory=True).numpy()
 
 
@@ -525,6 +529,12 @@ class DeviceSpecimen:
         if self._h:
             check(lib().totem_specimen_destroy(self._h))
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
