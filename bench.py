@@ -391,6 +391,11 @@ def main():
         kname, kavg, kbytes = "k_pass_cols<%d>" % N, cols_avg, bytes_cols
     achieved = kbytes / (kavg * 1e-3) / 1e9 if kavg > 0 else None
     traffic = measured_traffic(kname, N, args.workload)
+    # the other pass of the slice, and both together (per probe.slice: 32 N^2 bytes + shared slices)
+    oname, oavg, obytes = (("k_pass_cols<%d>" % N, cols_avg, bytes_cols) if kname.startswith("k_pass_rows")
+                           else ("k_pass_rows<%d>" % N, rows_avg, bytes_rows))
+    oach = obytes / (oavg * 1e-3) / 1e9 if oavg > 0 else None
+    both = (bytes_rows + bytes_cols) / ((rows_avg + cols_avg) * 1e-3) / 1e9 if rows_avg + cols_avg > 0 else None
 
     out = {
         "metric": "probe.slice/s at 1024^2 FP32 (batched vs plain one-probe-per-launch form)",
@@ -419,6 +424,11 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": kavg,
                      "rows_avg_ms": rows_avg, "cols_avg_ms": cols_avg},
+        "roofline_other_pass": {"kernel": oname, "achieved": oach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": (oach / HBM_PEAK_GBS) if oach else None,
+                                "traffic": measured_traffic(oname, N, args.workload), "avg_launch_ms": oavg},
+        "roofline_both_passes": {"achieved": both, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": (both / HBM_PEAK_GBS) if both else None},
     }
     if not args.no_cpu and args.cpu_probes > 0:
         cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
