@@ -120,6 +120,28 @@ def test_potential_matches_oracle(engine, flag, ions, absorption):
         assert et <= 1e-6
 
 
+def test_potential_with_per_atom_coefficients_matches_oracle(engine):
+    """Peng rows that all differ (per-atom Debye-Waller B folded into b, per-atom absorptive
+    coefficients, mixed ions): the planner splits them into per-Gaussian classes."""
+    print("case potential per-atom coefficients (Gaussian split)", flush=True)
+    from dataclasses import replace
+    cfg, inp, _ = _potential_case("p", ions=True)
+    rng = np.random.default_rng(8)
+    x = cfg.atoms.copy()
+    x[:, 6] = rng.uniform(0.2, 0.8, len(x))
+    cfg = replace(cfg, atoms=x, dw_selected=True)
+    inp = stem.prepare(cfg)
+    absorp = np.abs(rng.normal(0.0, 0.02, (len(inp.ele_n), 10))) * np.tile([1.0, 200.0], 5)
+    assert len({tuple(r[3:]) for r in inp.ele_n}) > len(inp.ele_n) // 2
+    V, t = _build_gpu(engine, inp, absorp)
+    Vo, to = _oracle_potential(inp, absorp)
+    for s in range(cfg.n_slices):
+        ev, et = _rel_l2(V[s], Vo[s]), _rel_l2(t[s], to[s])
+        print("  slice %d: V rel L2 %.3e  t rel L2 %.3e" % (s, ev, et))
+        assert ev <= 1e-6
+        assert et <= 1e-6
+
+
 def test_potential_only_entry_point_matches_oracle(engine):
     print("case waveCudaPengAndCorrP", flush=True)
     cfg, inp, _ = _potential_case("n")

@@ -178,7 +178,7 @@ struct PotentialPlan {
   std::vector<PotClass> classes;
 };
 
-static PotentialPlan plan_potential(const totem_potential_desc& d) {
+static PotentialPlan plan_potential(const totem_potential_desc& d, bool allow_split = true) {
   PotentialPlan P;
   const int np = d.np, S = d.slices;
   require(d.paraflag == 'l' || d.paraflag == 'p' || d.paraflag == 'n', "paraflag must be 'l', 'p' or 'n'");
@@ -287,6 +287,63 @@ static PotentialPlan plan_potential(const totem_potential_desc& d) {
         key[3] = check_cls(r[7]);
         P.slices[s].push_back({r[0], r[1], r[3], class_of(key)});
       }
+  }
+  // Rows with their own coefficient sets (erf-weighted slicing near faces,
+  // per-atom Debye-Waller factors folded into Peng's b) leave about one class per
+  // row, and every class costs a form-factor evaluation per pixel.  Then Peng
+  // rows are split into their Gaussian terms: a_q exp(-b_q s^2) becomes a class
+  // keyed by b_q alone (likewise C_q exp(-D_q s^2) of the absorptive part) with
+  // a_q moved into the term's weight.
+  size_t n_terms = 0;
+  for (const auto& sl : P.slices) n_terms += sl.size();
+  if (allow_split && P.classes.size() > 16 && P.classes.size() * 12 > n_terms) {
+    std::vector<ClassKeyV> key_of(P.classes.size());
+    for (const auto& kv : cmap) key_of[kv.second] = kv.first.v;
+    std::map<ClassKey, int> smap;
+    std::vector<PotClass> sclasses;
+    auto sclass = [&](const ClassKeyV& key) {
+      ClassKey k{key};
+      auto it = smap.find(k);
+      if (it != smap.end()) return it->second;
+      const int id = (int)sclasses.size();
+      sclasses.push_back(make_class(key));
+      smap.emplace(k, id);
+      return id;
+    };
+    for (auto& sl : P.slices) {
+      std::vector<Term> out;
+      out.reserve(sl.size() * 5);
+      for (const Term& t : sl) {
+        const PotClass& k = P.classes[t.cls];
+        if (k.kind != kPeng && k.kind != kPengIon) {
+          out.push_back({t.x, t.y, t.w, sclass(key_of[t.cls])});
+          continue;
+        }
+        auto gaussians = [&](const double* p, double f) {
+          for (int q = 0; q < 5; ++q) {
+            if (p[2 * q] == 0.0) continue;
+            ClassKeyV key = blank_key(kPeng);
+            key[7] = 1.0;
+            key[8] = p[2 * q + 1];
+            out.push_back({t.x, t.y, t.w * f * p[2 * q], sclass(key)});
+          }
+        };
+        gaussians(k.p1, k.kind == kPengIon ? k.w1 : 1.0);
+        if (k.kind == kPengIon) gaussians(k.p2, k.w2);
+        if (k.has_abs)
+          for (int q = 0; q < 5; ++q) {
+            if (k.ab[2 * q] == 0.0) continue;
+            ClassKeyV key = blank_key(kPeng);
+            key[1] = 1.0;
+            key[27] = 1.0;
+            key[28] = k.ab[2 * q + 1];
+            out.push_back({t.x, t.y, t.w * k.ab[2 * q], sclass(key)});
+          }
+      }
+      sl.swap(out);
+    }
+    P.classes.swap(sclasses);
+    cmap.swap(smap);
   }
   // Class ids in key order: any subset of the same classes keeps the same
   // relative order, so the per-slice segment order (and with it the summation
@@ -472,7 +529,7 @@ static totem_specimen* specimen_create_impl(const totem_specimen_desc& d) {
   require(d.n_terms >= 0 && (d.n_terms == 0 || d.term_atom), "specimen: term_atom missing");
   require(d.n_mid >= 0 && (d.n_mid == 0 || d.mid), "specimen: mid missing");
   require(d.multiceng >= 0 && std::isfinite(d.multiceng), "specimen: multiceng must be finite and >= 0");
-  PotentialPlan P = plan_potential(d.pot);
+  PotentialPlan P = plan_potential(d.pot, false);  // one term per row: term_atom indexes them
   const auto& terms = P.slices[0];
   require((long)terms.size() == (long)d.n_terms,
           "specimen: n_terms = " + std::to_string(d.n_terms) + " but pot holds " + std::to_string(terms.size()) +
