@@ -222,3 +222,15 @@ def test_amorphous_density_separation_and_seed():
     assert d[:, 1].min() >= 1.4
     np.testing.assert_array_equal(x, specimen.amorphous(30.0, 30.0, 5.0, 25.0, seed=7))
     assert not np.array_equal(x, specimen.amorphous(30.0, 30.0, 5.0, 25.0, seed=8))
+
+
+def test_bench_reads_committed_traffic_profile():
+    """bench.py takes the roofline's HBM traffic from the committed rocprof summary: both passes
+    parse, with read + write within 10% of the algorithmic bytes of a 26-probe launch."""
+    import bench
+    alg = 26 * 16.0 * 1024 * 1024 + 8.0 * 1024 * 1024
+    for k in ("k_pass_rows<1024>", "k_pass_cols<1024>"):
+        tr = bench.measured_traffic(k, 1024, "c2")
+        assert tr is not None, k
+        assert 0.95 * alg < tr < 1.10 * alg, (k, tr / alg)
+    assert bench.measured_traffic("k_pass_rows<512>", 512, "c2") is None
