@@ -22,7 +22,13 @@ semantics; the two agree whenever there are no mixed-ionicity atoms.
 Pins (``tests/test_oracle.py``): DC identity sum_r V dx^2 = 47.87298 * sum occ f(0);
 single Peng atom vs the closed-form projected potential (a sum of Gaussians);
 single Lobato atom vs its closed form (K0/K1 Bessel terms) away from the core;
-integer-pixel translation; Peng + (Lobato - Peng) correction == Lobato.
+integer-pixel translation; Peng + (Lobato - Peng) correction == Lobato;
+absorptive part: Im V of a Peng or Lobato atom (with and without ion rows) equals
+the Gaussian closed form of its C/D coefficients, undamped by Lobato's B, and
+leaves Re V unchanged; a fully ionic Peng atom equals the Gaussian closed form of
+its (Z, valence) row; mixed ionicity is linear in the ion weight
+(V(w) = w V(ion) + (1 - w) V(atom)) and matches the closed form; Lobato ion rows
+weight both coefficient sets and damp both by B.
 """
 import numpy as np
 

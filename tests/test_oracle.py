@@ -95,6 +95,127 @@ def test_lobato_debye_waller_factor():
     np.testing.assert_allclose(V1, V0 * np.exp(-0.6 * s2), rtol=1e-12, atol=1e-14)
 
 
+# Absorptive coefficients (C_q, D_q interleaved as absorp_n rows hold them); synthetic values
+# of the magnitude the reference's fitted tables have, D_q in Angstrom^2.
+ABSORP = np.array([0.021, 0.45, 0.048, 1.9, 0.093, 7.5, 0.031, 21.0, 0.012, 63.0])
+
+
+def _gauss_closed(c, r):
+    """Inverse transform of sum_q c_2q exp(-c_2q+1 s^2): C sum_q c_2q (4 pi/c_2q+1) exp(-4 pi^2 r^2/c_2q+1)."""
+    a, b = np.asarray(c)[0::2], np.asarray(c)[1::2]
+    return C_POT * np.sum(a * 4 * np.pi / b * np.exp(-4 * np.pi ** 2 * r[..., None] ** 2 / b), axis=-1)
+
+
+def _atom_potential(flag, np_side, dx, ele_n=None, absorp_n=None, ele_n_i=None, absorp_n_i=None):
+    """Real-space potential of one slice holding the given rows (complex; centred k grid)."""
+    gx, gy, s2 = optics.reciprocal_grids(np_side, dx)
+    par = optics.potential_parameter(np_side, dx)
+    lob = flag == "l"
+    e = np.zeros((0, 14 if lob else 13)) if ele_n is None else np.atleast_2d(ele_n)
+    ei = np.zeros((0, 26 if lob else 25)) if ele_n_i is None else np.atleast_2d(ele_n_i)
+    ab = None if absorp_n is None else np.atleast_2d(absorp_n)
+    abi = None if absorp_n_i is None else np.atleast_2d(absorp_n_i)
+    Vk = opot.reciprocal_potential(np_side, e, ab, ei, abi, np.array([len(e)], float), np.array([len(ei)], float),
+                                   s2, gx, gy, par, flag)
+    return opot.real_space_potential(Vk, np.ones_like(s2))[0]
+
+
+def _radius(np_side, dx, x0, y0):
+    yy, xx = np.mgrid[0:np_side, 0:np_side] * dx
+    return np.hypot(xx - x0, yy - y0)
+
+
+def _rel_max(V, Va, m):
+    return np.abs(V - Va)[m].max() / np.abs(Va[m]).max()
+
+
+def test_peng_absorptive_part_matches_gaussian_closed_form():
+    # ..._peng.m:41-53: V_k = P occ (sum a e^{-b s^2} + i sum C e^{-D s^2}) phase, so Re V is the
+    # elastic sum of Gaussians and Im V the absorptive one
+    np_side, dx, x0, y0 = 1024, 0.01, 5.133, 4.871
+    p = tables.interleaved(tables.peng_table()[37])
+    V = _atom_potential("p", np_side, dx, ele_n=np.concatenate([[x0, y0, 0.8], p]), absorp_n=ABSORP)
+    r = _radius(np_side, dx, x0, y0)
+    m = r < 3.0
+    assert _rel_max(V.real, 0.8 * _gauss_closed(p, r), m) < 2e-6
+    assert _rel_max(V.imag, 0.8 * _gauss_closed(ABSORP, r), m) < 2e-6
+
+
+def test_lobato_debye_waller_leaves_absorptive_part_undamped():
+    # ..._peng.m:120-133: exp(-B s^2) multiplies the Lobato elastic sum only; the absorptive
+    # Gaussians enter undamped, and they add nothing to Re V
+    np_side, dx, x0, y0, B = 1024, 0.01, 4.92, 5.31, 0.5
+    p = tables.interleaved(tables.lobato_table()[37])
+    row = np.concatenate([[x0, y0, 1.0], p, [B]])
+    V = _atom_potential("l", np_side, dx, ele_n=row, absorp_n=ABSORP)
+    V0 = _atom_potential("l", np_side, dx, ele_n=row)
+    r = _radius(np_side, dx, x0, y0)
+    m = r < 3.0
+    assert _rel_max(V.imag, _gauss_closed(ABSORP, r), m) < 2e-6
+    np.testing.assert_allclose(V.real, V0.real, rtol=0, atol=1e-12 * np.abs(V0.real).max())
+
+
+def test_fully_ionic_peng_atom_matches_gaussian_closed_form():
+    # ionicity 1 takes the Peng ion row of (Z, valence) (getpotentialpara_lotabo_peng.m:35-53)
+    np_side, dx, x0, y0 = 1024, 0.01, 5.07, 4.96
+    x = np.array([[38, x0, y0, 1.0, 2, 1.0, 0.0, 1.0]])
+    all_n, all_n_i = specimen.atom_parameters(x, "p")
+    ele_n, _, sn, _ = specimen.slice_atoms(all_n, all_n_i, 2.0, np.array([0.0]), 0, "p", 0.0)
+    assert sn.tolist() == [1.0]
+    V = _atom_potential("p", np_side, dx, ele_n=ele_n)
+    ion = tables.peng_ion_table()
+    c_ion = tables.interleaved(ion[(ion[:, 0] == 38) & (ion[:, 1] == 2)][0, 2:])
+    r = _radius(np_side, dx, x0, y0)
+    m = r < 3.0
+    assert _rel_max(V.real, _gauss_closed(c_ion, r), m) < 2e-6
+    # the neutral atom's potential is measurably different, so the check above discriminates
+    assert _rel_max(_gauss_closed(tables.interleaved(tables.peng_table()[37]), r), _gauss_closed(c_ion, r), m) > 1e-2
+
+
+@pytest.mark.parametrize("w", [0.3, 0.75])
+def test_mixed_ionicity_is_linear_in_the_ion_weight(w):
+    # ..._peng.m:58-99: occ (w f_ion + (1 - w) f_atom); the ion and atom slots of an ele_n_i row
+    # (getpotentialpara_lotabo_peng.m:59-86) must carry the ion and neutral coefficients
+    np_side, dx, x0, y0 = 1024, 0.01, 5.11, 5.02
+    Vs = {}
+    for ionicity in (w, 1.0, 0.0):
+        x = np.array([[22, x0, y0, 1.0, 4, ionicity, 0.0, 0.9]])
+        all_n, all_n_i = specimen.atom_parameters(x, "p")
+        ele_n, ele_n_i, sn, sni = specimen.slice_atoms(all_n, all_n_i, 2.0, np.array([0.0]), 0, "p", 0.0)
+        Vs[ionicity] = _atom_potential("p", np_side, dx, ele_n=ele_n if len(ele_n) else None,
+                                       ele_n_i=ele_n_i if len(ele_n_i) else None)
+    # ionicity 1 and 0 give single-form rows of occupancy 0.9 (ion and neutral coefficients),
+    # so V(w) = w V(1) + (1 - w) V(0); swapped weights or slots break it
+    ref = w * Vs[1.0] + (1 - w) * Vs[0.0]
+    np.testing.assert_allclose(Vs[w], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    ion = tables.peng_ion_table()
+    c_ion = tables.interleaved(ion[(ion[:, 0] == 22) & (ion[:, 1] == 4)][0, 2:])
+    c_atom = tables.interleaved(tables.peng_table()[21])
+    r = _radius(np_side, dx, x0, y0)
+    m = r < 3.0
+    closed = 0.9 * (w * _gauss_closed(c_ion, r) + (1 - w) * _gauss_closed(c_atom, r))
+    assert _rel_max(Vs[w].real, closed, m) < 2e-6
+
+
+def test_lobato_ion_rows_weight_both_sets_and_damp_them_by_b():
+    # ..._peng.m:141-177: occ ((w1 L1 + w2 L2) e^{-B s^2} + i sum C e^{-D s^2})
+    np_side, dx, x0, y0, B = 256, 0.04, 4.1, 5.3, 0.4
+    L1 = tables.interleaved(tables.lobato_table()[21])
+    L2 = tables.interleaved(tables.lobato_table()[7])
+    single = [_atom_potential("l", np_side, dx, ele_n=np.concatenate([[x0, y0, 0.7], L, [B]])) for L in (L1, L2)]
+    for w1, w2 in ((1.0, 0.0), (0.0, 1.0), (0.35, 0.65)):
+        row = np.concatenate([[x0, y0, 0.7, w1], L1, [w2], L2, [B]])
+        V = _atom_potential("l", np_side, dx, ele_n_i=row)
+        ref = w1 * single[0] + w2 * single[1]
+        np.testing.assert_allclose(V, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    # absorption on an ion row: Im V is the undamped Gaussian sum (fine grid: no Nyquist leak)
+    np_side, dx = 1024, 0.01
+    row = np.concatenate([[x0, y0, 0.7, 0.35], L1, [0.65], L2, [B]])
+    V = _atom_potential("l", np_side, dx, ele_n_i=row, absorp_n_i=ABSORP)
+    r = _radius(np_side, dx, x0, y0)
+    assert _rel_max(V.imag, 0.7 * _gauss_closed(ABSORP, r), r < 3.0) < 2e-6
+
+
 def test_peng_plus_correction_equals_lobato():
     a = 3.905
     x = specimen.perovskite(2, 2, 1, a=a)
