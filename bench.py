@@ -255,6 +255,8 @@ def main():
     ap.add_argument("--hrtem-images", type=int, default=256,
                     help="images per call of the batch HRTEM leg (0: skip)")
     ap.add_argument("--hrtem-only", action="store_true", help="run only the batch HRTEM leg (profiling)")
+    ap.add_argument("--only", choices=["stress", "uneven", "serving", "hrtem"],
+                    help="run only this leg and print it alone (profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-traffic kernel for "
                          "scaling hardware byte counters)")
@@ -269,8 +271,29 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     engine.lib()
+    if args.calibrate_copy_mib > 0:
+        src = torch.ones(args.calibrate_copy_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        torch.cuda.empty_cache()
     if args.hrtem_only:
+        args.only = "hrtem"
+    if args.only == "hrtem":
         print(json.dumps({"hrtem_batch": run_hrtem_batch(max(args.hrtem_images, 1), 4, seed=1006)}), flush=True)
+        return
+    if args.only == "stress":
+        print(json.dumps({"precision_stress": run_precision_stress(max(args.stress_scan, 1))}), flush=True)
+        return
+    if args.only == "uneven":
+        print(json.dumps({"uneven_batch": run_uneven(max(args.uneven_jobs, 1), seed=1004, arena_gib=0.25)}),
+              flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps({"serving": run_serving(max(args.serve_requests, 1), args.serve_rate, seed=1005,
+                                                 arena_gib=0.25, budget_gib=160)}), flush=True)
         return
     dev = torch.device("cuda:0")
     if args.workload == "c2":
@@ -309,13 +332,6 @@ def main():
     ws_bytes = engine.stem_workspace_bytes(job, batch)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
-    if args.calibrate_copy_mib > 0:
-        src = torch.ones(args.calibrate_copy_mib << 20, dtype=torch.uint8, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
 
     def build(r):
         if not args.host_slicing:

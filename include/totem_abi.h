@@ -108,6 +108,12 @@ typedef struct totem_potential_desc {
 int totem_build_transmission(const totem_potential_desc *desc, void *t_dev, double *v_re, double *v_im,
                              void *stream);
 
+/* Host-side plan of the same build, no device work: the form-factor classes, the
+ * terms over all slices, and split = 1 when Peng rows were split into one class per
+ * Gaussian term (done only when that lowers the accumulation's cost).  Diagnostic;
+ * no counterpart in the reference. */
+int totem_potential_plan_stats(const totem_potential_desc *desc, int *classes, long long *terms, int *split);
+
 /* --------------------------------------------------------- device STEM job
  * The propagation part of myProjectfun with every array already in HBM. */
 typedef struct totem_stem_job {

@@ -1,9 +1,9 @@
 """Transcribe the scattering-factor tables of the reference into CSV data files.
 
 Provenance script: it reads the reference's MATLAB table files as text and
-writes plain numeric CSV files next to itself.  It runs only where the
-reference checkout is present; the CSV files it produced are committed and are
-what the package loads.
+writes plain numeric CSV files into the package's data directory
+(totem_amd/data/).  It runs only where the reference checkout is present; the
+CSV files it produced are committed and are what the package loads.
 
 Sources (reference files, read as text):
   Lobato_para.m:4-311   -- Lobato & Van Dyck (2014) electron scattering factors,
@@ -43,7 +43,10 @@ def _numeric_rows(path, start_tag, stop_tag="];"):
     return rows
 
 
-def main(ref="/root/reference", out=os.path.dirname(os.path.abspath(__file__))):
+DATA = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))), "totem_amd", "data")
+
+
+def main(ref="/root/reference", out=DATA):
     lob = _numeric_rows(os.path.join(ref, "Lobato_para.m"), "para=")
     assert len(lob) == 206 and all(len(r) == 5 for r in lob), len(lob)
     lobato = [lob[2 * z] + lob[2 * z + 1] for z in range(103)]

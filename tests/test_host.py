@@ -234,3 +234,54 @@ def test_bench_reads_committed_traffic_profile():
         assert tr is not None, k
         assert 0.95 * alg < tr < 1.10 * alg, (k, tr / alg)
     assert bench.measured_traffic("k_pass_rows<512>", 512, "c2") is None
+
+
+# --------------------------------------------------------------------------- potential planner
+
+def _planner_case(multiceng=0, per_atom_dw=False, flag="p"):
+    from dataclasses import replace
+    a = 3.905
+    x = specimen.perovskite(3, 3, 3, a=a)
+    rng = np.random.default_rng(8)
+    x[:, 3] += rng.uniform(-0.3, 0.3, len(x))     # z off the lattice planes: many atoms near faces
+    dz = a / 2
+    cfg = stem.StemConfig(atoms=x, np_side=128, dx=3 * a / 128, n=64, kv=300.0, alpha_mrad=24.0,
+                          slice_thickness=dz, n_slices=6, slice_origin=-dz / 4, paraflag=flag, multiceng=multiceng)
+    if per_atom_dw:
+        x = x.copy()
+        x[:, 6] = rng.uniform(0.2, 0.8, len(x))
+        cfg = replace(cfg, atoms=x, dw_selected=True)
+    return stem.prepare(cfg)
+
+
+def _plan(inp):
+    from totem_amd import engine
+    c = inp.cfg
+    return engine.potential_plan_stats(c.paraflag, c.np_side, c.n_slices, inp.series_n, inp.ele_n, inp.gx, inp.gy,
+                                       inp.s2, inp.aper_true, inp.sigma, inp.parameter)
+
+
+def test_planner_splits_erf_weighted_rows_into_fewer_classes():
+    # multiceng = 1: rows near faces carry erf-scaled a_q (one coefficient set per row) but share
+    # their element's b_q, so one class per (element, Gaussian term) replaces one per row
+    inp = _planner_case(multiceng=1)
+    rows = len(inp.ele_n)
+    distinct_rows = len({tuple(r[3:]) for r in inp.ele_n})
+    st = _plan(inp)
+    assert distinct_rows > 16
+    assert st["split"]
+    assert st["classes"] <= 5 * 3 < distinct_rows        # Sr, Ti, O: five b_q each
+    assert rows < st["terms"] <= 5 * rows
+    # a static specimen sliced without erf weighting keeps one class per element
+    st0 = _plan(_planner_case(multiceng=0))
+    assert not st0["split"] and st0["classes"] == 3
+
+
+def test_planner_keeps_rows_whose_gaussians_do_not_repeat():
+    # per-atom Debye-Waller B folded into every b_q: splitting would make five classes per row
+    inp = _planner_case(per_atom_dw=True)
+    distinct_rows = len({tuple(r[3:]) for r in inp.ele_n})
+    assert distinct_rows == len(inp.ele_n)
+    st = _plan(inp)
+    assert not st["split"]
+    assert st["classes"] == distinct_rows and st["terms"] == len(inp.ele_n)

@@ -55,33 +55,14 @@ template <class T> static void upload(DevBuf& b, const std::vector<T>& v, hipStr
   if (!v.empty()) HIP_OK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
 }
 
-// Process-wide resident context: twiddle tables per (device, n) and one stream per device.
+// Process-wide resident context: one stream per device.
 struct Context {
   std::mutex mu;
-  std::map<std::pair<int, int>, void*> tw;
   std::map<int, hipStream_t> streams;
 };
 static Context& ctx() {
-  static Context* c = new Context();  // never destroyed: buffers live until process exit
+  static Context* c = new Context();  // never destroyed: streams live until process exit
   return *c;
-}
-
-const float2* tw_table(int n) {
-  int dev = 0;
-  HIP_OK(hipGetDevice(&dev));
-  Context& c = ctx();
-  std::lock_guard<std::mutex> lk(c.mu);
-  auto key = std::make_pair(dev, n);
-  auto it = c.tw.find(key);
-  if (it != c.tw.end()) return static_cast<const float2*>(it->second);
-  std::vector<float2> h;
-  twiddle_host(n, h);
-  require(!h.empty() || n <= 16, "no twiddle table for n=" + std::to_string(n), TOTEM_ERR_UNSUPPORTED);
-  void* p = nullptr;
-  HIP_OK(hipMalloc(&p, std::max<size_t>(h.size(), 1) * sizeof(float2)));
-  if (!h.empty()) HIP_OK(hipMemcpy(p, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
-  c.tw[key] = p;
-  return static_cast<const float2*>(p);
 }
 
 hipStream_t lib_stream() {
@@ -98,26 +79,32 @@ hipStream_t lib_stream() {
 }
 
 // The reference picks the GPU with the most free memory and makes it current
-// (cuda_source_lobato.cu:1063-1081).  TOTEM_DEVICE overrides the choice.
+// (cuda_source_lobato.cu:1063-1081), which creates a context on every visible
+// GPU.  Here the caller's current device is used unless TOTEM_DEVICE names one
+// (an index) or asks for the reference's choice ("max_free").
 static void select_device() {
   int count = 0;
   HIP_OK(hipGetDeviceCount(&count));
   require(count > 0, "no HIP device", TOTEM_ERR_DEVICE);
-  if (const char* env = std::getenv("TOTEM_DEVICE")) {
-    HIP_OK(hipSetDevice(std::atoi(env)));
+  const char* env = std::getenv("TOTEM_DEVICE");
+  if (!env || !*env) return;
+  if (std::strcmp(env, "max_free") != 0) {
+    char* end = nullptr;
+    const long d = std::strtol(env, &end, 10);
+    require(end && *end == 0 && d >= 0 && d < count,
+            std::string("TOTEM_DEVICE must be a device index below ") + std::to_string(count) + " or max_free");
+    HIP_OK(hipSetDevice((int)d));
     return;
   }
   int best = 0;
   size_t best_free = 0;
-  if (count > 1) {
-    for (int i = 0; i < count; ++i) {
-      HIP_OK(hipSetDevice(i));
-      size_t fr = 0, tot = 0;
-      HIP_OK(hipMemGetInfo(&fr, &tot));
-      if (fr > best_free) {
-        best_free = fr;
-        best = i;
-      }
+  for (int i = 0; i < count; ++i) {
+    HIP_OK(hipSetDevice(i));
+    size_t fr = 0, tot = 0;
+    HIP_OK(hipMemGetInfo(&fr, &tot));
+    if (fr > best_free) {
+      best_free = fr;
+      best = i;
     }
   }
   HIP_OK(hipSetDevice(best));
@@ -176,7 +163,28 @@ struct PotentialPlan {
   std::vector<double> kx1, ky1;
   std::vector<std::vector<Term>> slices;
   std::vector<PotClass> classes;
+  bool split = false;  // Peng rows split into per-Gaussian classes
 };
+
+// Relative cost of a plan's accumulation: per slice and class present, the
+// class's terms padded to the 16-term groups of the matrix-core sum, plus one
+// FP64 form-factor evaluation per pixel counted as kFormFactorTerms terms.
+constexpr long kFormFactorTerms = 32;
+static long plan_cost(const std::vector<std::vector<Term>>& slices, size_t n_classes) {
+  std::vector<int> cnt(n_classes, 0);
+  long cost = 0;
+  for (const auto& sl : slices) {
+    for (const Term& t : sl) ++cnt[t.cls];
+    for (const Term& t : sl) {
+      int& c = cnt[t.cls];
+      if (c > 0) {
+        cost += 16L * ((c + 15) / 16) + kFormFactorTerms;
+        c = 0;
+      }
+    }
+  }
+  return cost;
+}
 
 static PotentialPlan plan_potential(const totem_potential_desc& d, bool allow_split = true) {
   PotentialPlan P;
@@ -288,15 +296,15 @@ static PotentialPlan plan_potential(const totem_potential_desc& d, bool allow_sp
         P.slices[s].push_back({r[0], r[1], r[3], class_of(key)});
       }
   }
-  // Rows with their own coefficient sets (erf-weighted slicing near faces,
-  // per-atom Debye-Waller factors folded into Peng's b) leave about one class per
-  // row, and every class costs a form-factor evaluation per pixel.  Then Peng
-  // rows are split into their Gaussian terms: a_q exp(-b_q s^2) becomes a class
-  // keyed by b_q alone (likewise C_q exp(-D_q s^2) of the absorptive part) with
-  // a_q moved into the term's weight.
-  size_t n_terms = 0;
-  for (const auto& sl : P.slices) n_terms += sl.size();
-  if (allow_split && P.classes.size() > 16 && P.classes.size() * 12 > n_terms) {
+  // Rows with their own coefficient sets (erf-weighted slicing near faces scales
+  // Peng's a_q per row) leave about one class per row, and every class costs a
+  // form-factor evaluation per pixel.  Peng rows can then be split into their
+  // Gaussian terms: a_q exp(-b_q s^2) becomes a class keyed by b_q alone
+  // (likewise C_q exp(-D_q s^2) of the absorptive part) with a_q moved into the
+  // term's weight.  That helps only when the b_q repeat across rows; when they do
+  // not (a per-atom Debye-Waller B folded into every b_q) it multiplies the
+  // classes, so the split is kept only if it lowers plan_cost by a fifth.
+  if (allow_split && P.classes.size() > 16) {
     std::vector<ClassKeyV> key_of(P.classes.size());
     for (const auto& kv : cmap) key_of[kv.second] = kv.first.v;
     std::map<ClassKey, int> smap;
@@ -310,8 +318,10 @@ static PotentialPlan plan_potential(const totem_potential_desc& d, bool allow_sp
       smap.emplace(k, id);
       return id;
     };
-    for (auto& sl : P.slices) {
-      std::vector<Term> out;
+    std::vector<std::vector<Term>> sslices(P.slices.size());
+    for (size_t si = 0; si < P.slices.size(); ++si) {
+      const auto& sl = P.slices[si];
+      std::vector<Term>& out = sslices[si];
       out.reserve(sl.size() * 5);
       for (const Term& t : sl) {
         const PotClass& k = P.classes[t.cls];
@@ -340,10 +350,13 @@ static PotentialPlan plan_potential(const totem_potential_desc& d, bool allow_sp
             out.push_back({t.x, t.y, t.w * k.ab[2 * q], sclass(key)});
           }
       }
-      sl.swap(out);
     }
-    P.classes.swap(sclasses);
-    cmap.swap(smap);
+    if (5 * plan_cost(sslices, sclasses.size()) < 4 * plan_cost(P.slices, P.classes.size())) {
+      P.slices.swap(sslices);
+      P.classes.swap(sclasses);
+      cmap.swap(smap);
+      P.split = true;
+    }
   }
   // Class ids in key order: any subset of the same classes keeps the same
   // relative order, so the per-slice segment order (and with it the summation
@@ -434,6 +447,10 @@ static void flatten_plan(PotentialPlan& P, std::vector<double>& tx, std::vector<
   }
 }
 
+// Phase-table scratch (terms of the largest slice x np x 16 bytes) above this
+// size is released after the build instead of kept for the next one.
+constexpr size_t kKeepBuilderScratch = 256ull << 20;
+
 static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev, double* v_re, double* v_im,
                                     hipStream_t st) {
   PotentialPlan P = plan_potential(d);
@@ -491,6 +508,12 @@ static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev
   pd.parameter = d.parameter;
   potential_run(pd, t_dev, v_re, v_im, st, sc->dex, sc->dey);
   HIP_OK(hipStreamSynchronize(st));
+  // the phase tables grow with the terms of the largest slice; a large set is not
+  // kept for the rest of the process
+  if (sc->dex.bytes + sc->dey.bytes > kKeepBuilderScratch) {
+    sc->dex.release();
+    sc->dey.release();
+  }
 }
 
 // ------------------------------------------------------- specimen on the device
@@ -1241,6 +1264,18 @@ int totem_build_transmission(const totem_potential_desc* desc, void* t_dev, doub
     require(desc && t_dev, "desc and t_dev are required");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : lib_stream();
     build_transmission_impl(*desc, static_cast<float2*>(t_dev), v_re, v_im, st);
+  });
+}
+
+int totem_potential_plan_stats(const totem_potential_desc* desc, int* classes, long long* terms, int* split) {
+  return guarded([&] {
+    require(desc != nullptr, "desc is required");
+    PotentialPlan P = plan_potential(*desc);
+    long long n = 0;
+    for (const auto& sl : P.slices) n += (long long)sl.size();
+    if (classes) *classes = (int)P.classes.size();
+    if (terms) *terms = n;
+    if (split) *split = P.split ? 1 : 0;
   });
 }
 

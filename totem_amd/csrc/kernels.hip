@@ -5,18 +5,22 @@
 //                    [annular detectors] -> inverse column FFT   (exit slice: detectors only)
 //  * k_reduce_det -- per finished probe: fixed-order compensated FP64 sum of detector partials
 //  * k_phase / k_accumulate_mfma16 -- projected potential in reciprocal space per slice
-//                    (structure-factor sums on the matrix cores; k_accumulate_mfma
-//                    (32 x 32 tiles) and k_accumulate (VALU) are other forms of the same sum)
+//                    (structure-factor sums on the matrix cores, 16 x 16 pixel tiles)
 //  * k_fft_rows / k_fft_cols -- plain batched 2-D FFT passes (potential IFFT, tests)
 //
 // A full 2-D forward+inverse FFT per slice therefore costs two read+write
 // passes over the probe wave in HBM (32 N^2 bytes per probe.slice).
 #include "fft_block.h"
+#include "host_common.h"
 #include "kernels.h"
 #include "launch.h"
 #include "unit_round.h"
 
+#include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <string>
 
 // Tuning knobs (compile-time).
 #ifndef TOTEM_ROWS_MINWAVES
@@ -25,11 +29,20 @@
 #ifndef TOTEM_COLS_MINWAVES
 #define TOTEM_COLS_MINWAVES 1
 #endif
+// Columns per columns-pass workgroup.  The workgroup holds C whole columns in LDS
+// ((N + N/16) * C float2 with the bank padding), which must stay within 64 KiB:
+// 16 columns of 512 points or 4 of 2048 would need 68 KiB.
+#ifndef TOTEM_COLS_C512
+#define TOTEM_COLS_C512 8
+#endif
 #ifndef TOTEM_COLS_C1024
-#define TOTEM_COLS_C1024 4     // columns per workgroup of the 1024-point columns pass
+#define TOTEM_COLS_C1024 4
 #endif
 #ifndef TOTEM_COLS_C1536
-#define TOTEM_COLS_C1536 4     // ... and of the 1536- and 2048-point ones
+#define TOTEM_COLS_C1536 4
+#endif
+#ifndef TOTEM_COLS_C2048
+#define TOTEM_COLS_C2048 2
 #endif
 #ifndef TOTEM_ROWS_PP
 #define TOTEM_ROWS_PP 0        // rows pass: ping-pong LDS (1) or one buffer (0)
@@ -73,9 +86,19 @@ template <int N> struct ColCfg {
   static constexpr int NBUF = PP ? 2 : 1;
   static constexpr int T = Plan<N>::T;
   // columns per workgroup
-  static constexpr int C = (N <= 512) ? 16 : (N >= 1536 ? TOTEM_COLS_C1536 : (N == 1024 ? TOTEM_COLS_C1024 : 8));
+  static constexpr int C = N < 512 ? 16
+                           : N == 512 ? TOTEM_COLS_C512
+                           : N == 1024 ? TOTEM_COLS_C1024
+                           : N == 1536 ? TOTEM_COLS_C1536
+                           : N == 2048 ? TOTEM_COLS_C2048 : 8;
   static constexpr int THREADS = T * C;
 };
+// Every pass kernel's static LDS (transform buffers plus the detector's wave sums)
+// stays within 64 KiB per workgroup.
+#ifndef TOTEM_LDS_LIMIT
+#define TOTEM_LDS_LIMIT (64 * 1024)
+#endif
+constexpr int kLdsLimit = TOTEM_LDS_LIMIT;
 
 #ifndef TOTEM_TILED
 #define TOTEM_TILED 1  // in-flight waves in the tiled layout below (0: row-major)
@@ -221,6 +244,7 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   using Cfg = RowCfg<N>;
   constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, K = TOTEM_ROWS_K;
   constexpr int LLP = Cfg::LL + rows_lds_pad<N>();
+  static_assert(Cfg::NBUF * RW * LLP * sizeof(float2) <= kLdsLimit, "rows-pass LDS above 64 KiB");
   __shared__ float2 lds[Cfg::NBUF * RW * LLP];
   float2* const lds1 = lds + (Cfg::NBUF - 1) * RW * LLP;
   int lr, t;
@@ -360,6 +384,8 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C, K = TOTEM_COLS_K;
   constexpr int NW = (Cfg::THREADS + 63) / 64;
   constexpr int LC = col_lds_len<N, C>();
+  static_assert(Cfg::NBUF * LC * sizeof(float2) + NW * kMaxAper * sizeof(double) <= kLdsLimit,
+                "columns-pass LDS above 64 KiB");
   __shared__ float2 lds[Cfg::NBUF * LC];
   __shared__ double red[NW * kMaxAper];
   float2* const lds1 = lds + (Cfg::NBUF - 1) * LC;
@@ -470,6 +496,7 @@ template <int N, int SIGN>
 __global__ void __launch_bounds__(RowCfg<N>::THREADS) k_fft_rows(float2* __restrict__ data, long rows, const float2* __restrict__ tw) {
   using Cfg = RowCfg<N>;
   constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T;
+  static_assert(RW * Cfg::LL * sizeof(float2) <= kLdsLimit, "row FFT LDS above 64 KiB");
   __shared__ float2 lds[RW * Cfg::LL];
   const int lr = threadIdx.x / T;
   const int t = threadIdx.x - lr * T;
@@ -493,6 +520,7 @@ k_fft_cols(const float2* __restrict__ src, float2* __restrict__ dst, int epi, fl
            const float2* __restrict__ tw) {
   using Cfg = ColCfg<N>;
   constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C;
+  static_assert(col_lds_len<N, C>() * sizeof(float2) <= kLdsLimit, "column FFT LDS above 64 KiB");
   __shared__ float2 lds[col_lds_len<N, C>()];
   const int mat = blockIdx.x / G;
   const int cg = blockIdx.x - mat * G;
@@ -579,143 +607,22 @@ __device__ __forceinline__ double2 form_factor(const PotClass& k, double s2, con
 
 // One slice: V_k(r,c) = PARAMETER * AperTrue * sum_classes f_cls(k) * sum_{j in cls} w_j ex_j[c] ey_j[r],
 // written at the ifftshifted position (the reference's quadrant swap) as complex64.
-constexpr int kAccTile = 16;
-constexpr int kAccChunk = 64;
-__global__ void __launch_bounds__(256)
-k_accumulate(const float2* __restrict__ ex, const float2* __restrict__ ey, const double* __restrict__ wt,
-             const PotSeg* __restrict__ segs, int n_segs, const PotClass* __restrict__ classes,
-             const double* __restrict__ s2, const double* __restrict__ aper_true, const double* __restrict__ corr,
-             double parameter, int np, float2* __restrict__ vshift) {
-  __shared__ float2 sx[kAccChunk][kAccTile];
-  __shared__ float2 sy[kAccChunk][kAccTile];
-  __shared__ float sw[kAccChunk];
-  const int lc = threadIdx.x % kAccTile, lr = threadIdx.x / kAccTile;
-  const int c0 = blockIdx.x * kAccTile, r0 = blockIdx.y * kAccTile;
-  const int c = c0 + lc, r = r0 + lr;
-  const bool inside = (r < np) && (c < np);
-  const size_t pix = (size_t)r * np + c;
-  const size_t np2 = (size_t)np * np;
-  const double s2v = inside ? s2[pix] : 0.0;
-  double vr = 0.0, vi = 0.0;
-  for (int sgi = 0; sgi < n_segs; ++sgi) {
-    const PotSeg sg = segs[sgi];
-    float ar = 0.f, ai = 0.f;
-    for (int base = sg.begin; base < sg.end; base += kAccChunk) {
-      const int cnt = min(kAccChunk, sg.end - base);
-      __syncthreads();
-      for (int i = threadIdx.x; i < kAccChunk * kAccTile; i += blockDim.x) {
-        const int jj = i / kAccTile, q = i % kAccTile;
-        if (jj < cnt) {
-          const size_t row = (size_t)(base + jj) * np;
-          sx[jj][q] = (c0 + q < np) ? ex[row + c0 + q] : make_float2(0.f, 0.f);
-          sy[jj][q] = (r0 + q < np) ? ey[row + r0 + q] : make_float2(0.f, 0.f);
-        }
-      }
-      for (int i = threadIdx.x; i < cnt; i += blockDim.x) sw[i] = (float)wt[base + i];
-      __syncthreads();
-      for (int jj = 0; jj < cnt; ++jj) {
-        const float2 p = cmul(sx[jj][lc], sy[jj][lr]);
-        ar = fmaf(sw[jj], p.x, ar);
-        ai = fmaf(sw[jj], p.y, ai);
-      }
-    }
-    if (inside) {
-      const double2 f = form_factor(classes[sg.cls], s2v, corr, pix, np2);
-      vr += f.x * ar - f.y * ai;
-      vi += f.x * ai + f.y * ar;
-    }
-  }
-  if (inside) {
-    const double scale = parameter * aper_true[pix];
-    const int half = np / 2;
-    const int rs = (r + half) % np, cs = (c + half) % np;
-    vshift[(size_t)rs * np + cs] = make_float2((float)(vr * scale), (float)(vi * scale));
-  }
-}
-
-// The same sum as a complex matrix product on the matrix cores: per class,
-// A[r][j] = ey_j[r] (32 rows x 2 terms) times B[j][c] = w_j ex_j[c] (2 terms x 32
-// columns) accumulates a 32 x 32 pixel tile in FP32 with four real
-// v_mfma_f32_32x32x2f32 per pair of terms (Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br);
-// the tile is then weighted by the class's FP64 form factor per pixel.  One wave
-// per tile.  Operand layout of the 32x32x2 instruction: lane l supplies
-// A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator element i of lane l is
-// C[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
-#ifndef TOTEM_ACC_MFMA
-#define TOTEM_ACC_MFMA 2  // 2: 16 x 16 tiles (measured faster: 36.7 vs 44.1 ms per 80-slice build), 1: 32 x 32, 0: VALU
-#endif
-constexpr int kMfmaTile = 32;
+// Per class the sum over terms is a complex matrix product, A[r][j] = ey_j[r]
+// times B[j][c] = w_j ex_j[c], accumulated on the matrix cores in FP32; the tile
+// is then weighted by the class's FP64 form factor per pixel.  One wave per
+// 16 x 16 pixel tile (every supported grid side is a multiple of 16).
 #ifndef TOTEM_ACC_PAIRS
 #define TOTEM_ACC_PAIRS 4
 #endif
 constexpr int kMfmaPairs = TOTEM_ACC_PAIRS;
-typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-__global__ void __launch_bounds__(64)
-k_accumulate_mfma(const float2* __restrict__ ex, const float2* __restrict__ ey, const double* __restrict__ wt,
-                  const PotSeg* __restrict__ segs, int n_segs, const PotClass* __restrict__ classes,
-                  const double* __restrict__ s2, const double* __restrict__ aper_true,
-                  const double* __restrict__ corr, double parameter, int np, float2* __restrict__ vshift) {
-  const int lane = threadIdx.x;
-  const int tiles = np / kMfmaTile;
-  const int tr = blockIdx.x / tiles, tc = blockIdx.x - tr * tiles;
-  const int r0 = tr * kMfmaTile, c0 = tc * kMfmaTile;
-  const int li = lane & 31, lk = lane >> 5;
-  const size_t np2 = (size_t)np * np;
-  double vr[16], vi[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) vr[i] = vi[i] = 0.0;
-  for (int sgi = 0; sgi < n_segs; ++sgi) {
-    const PotSeg sg = segs[sgi];
-    floatx16 cr = {}, ci = {};
-    // kMfmaPairs pairs of terms per step: all their loads are issued before the
-    // first product, so one wait covers the latency of the whole group
-    for (int j = sg.begin; j < sg.end; j += 2 * kMfmaPairs) {
-      float2 a[kMfmaPairs], b[kMfmaPairs];
-#pragma unroll
-      for (int u = 0; u < kMfmaPairs; ++u) {
-        const int jj = j + 2 * u + lk;
-        a[u] = b[u] = make_float2(0.f, 0.f);
-        if (jj < sg.end) {
-          a[u] = ey[(size_t)jj * np + r0 + li];
-          const float w = (float)wt[jj];
-          const float2 e = ex[(size_t)jj * np + c0 + li];
-          b[u] = make_float2(w * e.x, w * e.y);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kMfmaPairs; ++u) {
-        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u].x, b[u].x, cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u].x, b[u].y, ci, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a[u].y, b[u].y, cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u].y, b[u].x, ci, 0, 0, 0);
-      }
-    }
-    const PotClass& cls = classes[sg.cls];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const int r = r0 + 8 * (i / 4) + 4 * lk + (i % 4), c = c0 + li;
-      const size_t pix = (size_t)r * np + c;
-      const double2 f = form_factor(cls, s2[pix], corr, pix, np2);
-      vr[i] += f.x * cr[i] - f.y * ci[i];
-      vi[i] += f.x * ci[i] + f.y * cr[i];
-    }
-  }
-  const int half = np / 2;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int r = r0 + 8 * (i / 4) + 4 * lk + (i % 4), c = c0 + li;
-    const size_t pix = (size_t)r * np + c;
-    const double scale = parameter * aper_true[pix];
-    const int rs = (r + half) % np, cs = (c + half) % np;
-    vshift[(size_t)rs * np + cs] = make_float2((float)(vr[i] * scale), (float)(vi[i] * scale));
-  }
-}
-
-// 16 x 16 pixel tiles with v_mfma_f32_16x16x4f32 (4 terms per product): lane l
-// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator element i of
-// lane l is C[4 (l / 16) + i][l % 16].  Fewer registers per wave than the
-// 32 x 32 form, at twice the operand reads per pixel.
+// v_mfma_f32_16x16x4f32 (4 terms per product, four real products per complex
+// one: Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br): lane l supplies A[l % 16][l / 16]
+// and B[l / 16][l % 16]; accumulator element i of lane l is C[4 (l / 16) + i][l % 16].
+// All loads of kMfmaPairs groups of four terms are issued before their products.
+// Measured per 80-slice 1024^2 build of 2000 terms per slice: 36.7 ms, against
+// 44.1 ms with 32 x 32 tiles (v_mfma_f32_32x32x2f32, 186 VGPRs, 2 waves/SIMD)
+// and 54.4 ms with VALU outer products over LDS-staged terms.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(64)
 k_accumulate_mfma16(const float2* __restrict__ ex, const float2* __restrict__ ey, const double* __restrict__ wt,
@@ -842,6 +749,51 @@ void twiddle_host(int n, std::vector<float2>& out) {
 #undef X
 }
 
+// Every plan's twiddle table lives in one static device array (no allocation on
+// any path that launches the passes); each table starts at an even entry so that
+// its float4 reads stay 16-byte aligned.
+constexpr int tw_even(int n) { return (n + 1) & ~1; }
+constexpr int kTwTotal = 0
+#define X(S) + tw_even(twiddle_len<S>())
+    TOTEM_FOR_SIZES(X)
+#undef X
+    ;
+__device__ __align__(16) float2 g_twiddles[kTwTotal];
+
+static int twiddle_offset(int n) {
+  int off = 0;
+#define X(S) if (n == S) return off; off += tw_even(twiddle_len<S>());
+  TOTEM_FOR_SIZES(X)
+#undef X
+  return -1;
+}
+
+const float2* tw_table(int n) {
+  const int off = twiddle_offset(n);
+  require(off >= 0, "no FFT plan for n=" + std::to_string(n), TOTEM_ERR_UNSUPPORTED);
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::map<int, float2*> base;  // per device: address of g_twiddles, filled once
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = base.find(dev);
+  if (it == base.end()) {
+    float2* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(g_twiddles)));
+    std::vector<float2> all(kTwTotal, make_float2(0.f, 0.f)), h;
+    int o = 0;
+#define X(S)                                                   \
+  twiddle_host_n<S>(h);                                        \
+  std::copy(h.begin(), h.end(), all.begin() + o);              \
+  o += tw_even(twiddle_len<S>());
+    TOTEM_FOR_SIZES(X)
+#undef X
+    HIP_OK(hipMemcpy(p, all.data(), all.size() * sizeof(float2), hipMemcpyHostToDevice));
+    it = base.emplace(dev, p).first;
+  }
+  return it->second + off;
+}
+
 bool size_supported(int n) {
 #define X(S) if (n == S) return true;
   TOTEM_FOR_SIZES(X)
@@ -899,21 +851,12 @@ void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_term
 void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,
                 const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
                 double parameter, int np, float2* vshift) {
-  if (TOTEM_ACC_MFMA == 2 && np % 16 == 0) {
-    const int tiles = np / 16;
-    hipLaunchKernelGGL(k_accumulate_mfma16, dim3((unsigned)(tiles * tiles)), dim3(64), 0, st, ex, ey, wt, segs,
-                       n_segs, classes, s2, aper_true, corr, parameter, np, vshift);
-    return;
-  }
-  if (TOTEM_ACC_MFMA && np % kMfmaTile == 0) {
-    const int tiles = np / kMfmaTile;
-    hipLaunchKernelGGL(k_accumulate_mfma, dim3((unsigned)(tiles * tiles)), dim3(64), 0, st, ex, ey, wt, segs,
-                       n_segs, classes, s2, aper_true, corr, parameter, np, vshift);
-    return;
-  }
-  const dim3 grid((np + kAccTile - 1) / kAccTile, (np + kAccTile - 1) / kAccTile);
-  hipLaunchKernelGGL(k_accumulate, grid, dim3(256), 0, st, ex, ey, wt, segs, n_segs, classes, s2, aper_true, corr,
-                     parameter, np, vshift);
+#define X(S) static_assert(S % 16 == 0, "potential tiles need grid sides that are multiples of 16");
+  TOTEM_FOR_SIZES(X)
+#undef X
+  const int tiles = np / 16;
+  hipLaunchKernelGGL(k_accumulate_mfma16, dim3((unsigned)(tiles * tiles)), dim3(64), 0, st, ex, ey, wt, segs, n_segs,
+                     classes, s2, aper_true, corr, parameter, np, vshift);
 }
 
 }  // namespace totem

@@ -34,6 +34,7 @@ EXPORTS = (
     "myProjectfun",
     "totem_project_lobato", "totem_project_peng", "totem_potential_lobato", "totem_potential_peng",
     "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
+    "totem_potential_plan_stats",
     "totem_stem_workspace_bytes", "totem_stem_run", "totem_kernel_times", "totem_fft2d",
     "totem_stem_preferred_batch", "totem_stem_autotune", "totem_hrtem_images", "totem_tcc_image", "totem_weighted_pattern",
     "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
@@ -265,6 +266,20 @@ def build_transmission(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx,
         vim = np.zeros((slices, np_side, np_side))
     check(lib().totem_build_transmission(ctypes.byref(d), _VP(t_dev_ptr), _dp(vre), _dp(vim), _VP(stream)))
     return (vre, vim) if want_v else None
+
+
+def potential_plan_stats(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx, ky, s_2, aper_true, sigma,
+                         parameter, **kw):
+    """Host-side plan of build_transmission's arguments (no device work): a dict with the
+    form-factor ``classes``, the ``terms`` over all slices and ``split`` (Peng rows split into
+    one class per Gaussian term)."""
+    d, keep = _potential_desc(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx, ky, s_2, aper_true, sigma,
+                              parameter, **kw)
+    classes, split = ctypes.c_int(0), ctypes.c_int(0)
+    terms = ctypes.c_longlong(0)
+    check(lib().totem_potential_plan_stats(ctypes.byref(d), ctypes.byref(classes), ctypes.byref(terms),
+                                           ctypes.byref(split)))
+    return {"classes": classes.value, "terms": terms.value, "split": bool(split.value)}
 
 
 def _potential_desc(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx, ky, s_2, aper_true, sigma, parameter,

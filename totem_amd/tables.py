@@ -1,7 +1,7 @@
 """Electron scattering-factor parameter tables (FP64).
 
 The numbers are the published parameterisations the reference ships as MATLAB
-data files, transcribed to CSV by ``data/extract_tables.py``:
+data files, transcribed to CSV by ``tests/golden/extract_tables.py``:
 
 * Lobato & Van Dyck (2014), ``Lobato_para.m:4-315``: f(g) = sum_i a_i (2 + b_i g^2) / (1 + b_i g^2)^2,
   g in 1/Angstrom, Z = 1..103, stored as [a1..a5, b1..b5].
