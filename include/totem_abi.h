@@ -163,6 +163,19 @@ size_t totem_stem_workspace_bytes(const totem_stem_job *job, int batch);
 int totem_stem_run(const totem_stem_job *job, int batch, int flags, void *workspace, size_t ws_bytes,
                    void *stream);
 
+/* totem_stem_run without any device allocation (no hipMalloc/hipFree, no allocating
+ * library call).  No counterpart in the reference: for callers that cannot allocate
+ * device memory (serving loops and harnesses that forbid hipMalloc).  The workspace
+ * (waves, detector partials, work items) is the library's static device array of
+ * TOTEM_STATIC_SCRATCH_BYTES per GPU, held from the library's first use on that GPU
+ * until the process ends; a job whose totem_stem_workspace_bytes(job, batch) exceeds
+ * it is refused with TOTEM_ERR_MEMORY.  Every call on one GPU shares that array and
+ * it keeps the last call's contents: calls that overlap on one GPU (other host
+ * threads) race.  Plain launches only (no graph, no timing).  Returns when the run
+ * has finished on `stream` (NULL: the device's null stream). */
+#define TOTEM_STATIC_SCRATCH_BYTES (128u << 20)
+int totem_stem_run_static(const totem_stem_job *job, int batch, void *stream);
+
 /* Accumulated durations (ms) and launch counts of the rows and columns passes
  * over TOTEM_RUN_TIMING runs on the calling thread; reset != 0 clears them after reading. */
 void totem_kernel_times(int reset, double *rows_ms, long *rows_n, double *cols_ms, long *cols_n);

@@ -986,6 +986,49 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   if (own.p) HIP_OK(hipStreamSynchronize(st));
 }
 
+// Workspace of totem_stem_run_static: one array per GPU, in the library's code
+// object, held from the library's first use on that GPU until the process ends.
+// Host code never uses the array's name as an address (on the host it is only a
+// stand-in for the device symbol): static_scratch() looks up each device's own
+// copy once with hipGetSymbolAddress and every launch and copy uses that pointer.
+// Nothing in it must start at zero: every wave slot is written at slice 0 and
+// every detector partial a reduction reads is written earlier in the same run.
+static __device__ __align__(256) unsigned char scratch[TOTEM_STATIC_SCRATCH_BYTES];
+
+static void* static_scratch() {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::map<int, void*> ptr;  // device -> address of its scratch
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = ptr.find(dev);
+  if (it != ptr.end()) return it->second;
+  void* p = nullptr;
+  HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  require(p != nullptr, "static scratch has no device address", TOTEM_ERR_DEVICE);
+  ptr.emplace(dev, p);
+  return p;
+}
+
+// totem_stem_run without allocation: waves, detector partials, work items and the
+// job entry in the static scratch; synchronous, so that sequential calls from any
+// streams never share the array while a run is in flight.
+static void run_stem_static_impl(const totem_stem_job& j, int batch, hipStream_t st) {
+  validate_job(j);
+  require(batch > 0, "batch must be positive");
+  const long total = (long)j.n_probes * j.depth;
+  batch = (int)std::min<long>(batch, total);
+  const size_t need = stem_ws_bytes(j.n, j.slices, j.n_aper, std::max(j.n_mid, 0), batch, total);
+  require(need <= (size_t)TOTEM_STATIC_SCRATCH_BYTES,
+          "job needs " + std::to_string(need) + " bytes of workspace for " + std::to_string(batch) +
+              " probes in flight; the static scratch holds " + std::to_string((size_t)TOTEM_STATIC_SCRATCH_BYTES) +
+              " (use fewer probes per launch or totem_stem_run)",
+          TOTEM_ERR_MEMORY);
+  void* ws = static_scratch();
+  run_stem_impl(j, batch, 0, ws, (size_t)TOTEM_STATIC_SCRATCH_BYTES, st);
+  HIP_OK(hipStreamSynchronize(st));
+}
+
 // Probes per launch by measurement: a synthetic job (unit transmission and probe,
 // one detector) of 2 x b probes through `slices` slices for b around the cache-fit
 // rule; the fastest per probe.slice is kept for (device, n).
@@ -1364,6 +1407,13 @@ int totem_stem_run(const totem_stem_job* job, int batch, int flags, void* worksp
     require(job != nullptr, "job is required");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : lib_stream();
     run_stem_impl(*job, batch, flags, workspace, ws_bytes, st);
+  });
+}
+
+int totem_stem_run_static(const totem_stem_job* job, int batch, void* stream) {
+  return guarded([&] {
+    require(job != nullptr, "job is required");
+    run_stem_static_impl(*job, batch, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -35,7 +35,7 @@ EXPORTS = (
     "totem_project_lobato", "totem_project_peng", "totem_potential_lobato", "totem_potential_peng",
     "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
     "totem_potential_plan_stats",
-    "totem_stem_workspace_bytes", "totem_stem_run", "totem_kernel_times", "totem_fft2d",
+    "totem_stem_workspace_bytes", "totem_stem_run", "totem_stem_run_static", "totem_kernel_times", "totem_fft2d",
     "totem_stem_preferred_batch", "totem_stem_autotune", "totem_hrtem_images", "totem_tcc_image", "totem_weighted_pattern",
     "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
     "totem_server_destroy",
@@ -132,6 +132,8 @@ def lib():
         L.totem_stem_workspace_bytes.restype = ctypes.c_size_t
         L.totem_stem_run.argtypes = [ctypes.POINTER(StemJob), _I, _I, _VP, ctypes.c_size_t, _VP]
         L.totem_stem_run.restype = _I
+        L.totem_stem_run_static.argtypes = [ctypes.POINTER(StemJob), _I, _VP]
+        L.totem_stem_run_static.restype = _I
         L.totem_kernel_times.argtypes = [_I, _DP, ctypes.POINTER(ctypes.c_long), _DP, ctypes.POINTER(ctypes.c_long)]
         L.totem_kernel_times.restype = None
         L.totem_fft2d.argtypes = [_VP, _I, _I, _I, _VP]
@@ -411,6 +413,15 @@ def stem_run(job, batch, graph=False, workspace_ptr=None, workspace_bytes=0, str
     flags = (TOTEM_RUN_GRAPH if graph else 0) | (TOTEM_RUN_TIMING if timing else 0)
     check(lib().totem_stem_run(ctypes.byref(job), int(batch), flags, _VP(workspace_ptr),
                                ctypes.c_size_t(workspace_bytes), _VP(stream)))
+
+
+STATIC_SCRATCH_BYTES = 128 << 20   # TOTEM_STATIC_SCRATCH_BYTES
+
+
+def stem_run_static(job, batch, stream=None):
+    """totem_stem_run_static: the run in the library's static device scratch, no allocation;
+    returns when the run has finished."""
+    check(lib().totem_stem_run_static(ctypes.byref(job), int(batch), _VP(stream)))
 
 
 def kernel_times(reset=True):
