@@ -229,7 +229,11 @@ template <bool WAVE> __device__ __forceinline__ void lds_sync() {
   }
 }
 
-template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, int... RS> struct Stages;
+// SPLIT: every exchange runs in two rounds through an LDS buffer of N/2 elements
+// per transform (elements [0, N/2) first, then [N/2, N)), for lengths whose full
+// exchange buffer would not fit.  A lane reads elements t + T*v, which lie in the
+// lower half exactly for v < E/2; the writes of a round are predicated per element.
+template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, bool SPLIT, int... RS> struct Stages;
 
 // Twiddles of the stage (P, R) for the U butterflies of lane t: block k = b mod P
 // of the stage table, R entries (index u*R + q; entry 0 is 1).  A stage's loads
@@ -266,21 +270,23 @@ template <int T, int P, int R, int U> struct StageTw {
   }
 };
 
-template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING> struct Stages<N, T, SIGN, Lay, PP, P, PING> {
+template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, bool SPLIT>
+struct Stages<N, T, SIGN, Lay, PP, P, PING, SPLIT> {
   using Tw = StageTw<T, 1, 1, 1>;
   template <class W>
   static __device__ __forceinline__ void run(float2*, float2*, float2*, const Lay&, int, const float2*, const W&) {}
 };
 
-template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, int R, int... RS>
-struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
+template <int N, int T, int SIGN, class Lay, bool PP, int P, int PING, bool SPLIT, int R, int... RS>
+struct Stages<N, T, SIGN, Lay, PP, P, PING, SPLIT, R, RS...> {
   static constexpr int E = N / T;
   static constexpr int U = E / R;
   static_assert(E % R == 0, "radix must divide elements per lane");
   static_assert(T % 16 == 0, "lane count must be a multiple of 16");
+  static_assert(!SPLIT || (!PP && E % 2 == 0 && (N / 2) % 16 == 0), "split exchange: one buffer, even E");
   static constexpr bool LAST = sizeof...(RS) == 0;
   using Tw = StageTw<T, P, R, U>;
-  using Next = Stages<N, T, SIGN, Lay, PP, P * R, PP ? 1 - PING : 0, RS...>;
+  using Next = Stages<N, T, SIGN, Lay, PP, P * R, PP ? 1 - PING : 0, SPLIT, RS...>;
 
   // `tw` points at this stage's table; `cur` holds this stage's twiddles.
   static __device__ __forceinline__ void run(float2* x, float2* lds0, float2* lds1, const Lay& lay, int t,
@@ -290,6 +296,7 @@ struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
     typename Next::Tw nxt;
     if constexpr (!LAST && !TOTEM_TW_LATE) nxt.load(tw_next, t);
     float2 out[E];
+    int jw[U];  // exchange position of each butterfly's output 0 (split exchanges)
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int b = t + T * u;
@@ -306,7 +313,8 @@ struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
         }
       }
       dft<R, SIGN>(a);
-      if constexpr (LAST) {
+      if constexpr (LAST || SPLIT) {
+        jw[u] = (b - k) * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) out[u + r * U] = a[r];
       } else {
@@ -324,6 +332,35 @@ struct Stages<N, T, SIGN, Lay, PP, P, PING, R, RS...> {
     if constexpr (LAST) {
 #pragma unroll
       for (int v = 0; v < E; ++v) x[v] = out[v];
+    } else if constexpr (SPLIT) {
+      if constexpr (TOTEM_TW_LATE) nxt.load(tw_next, t);
+      constexpr int H = N / 2;
+      // a butterfly's outputs jw + r P (r < R) lie in one aligned block of P R
+      // elements, and P R divides N/2: each butterfly writes in one round only
+      static_assert(H % (P * R) == 0, "split exchange: butterfly blocks must not straddle N/2");
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if ((jw[u] >= H) == (h == 1)) {
+            const int j = jw[u] - h * H;
+            if constexpr (Lay::template linear_write<P, R>()) {
+              float2* wp = buf + lay(j);
+#pragma unroll
+              for (int r = 0; r < R; ++r) wp[Lay::off(r * P)] = out[u + r * U];
+            } else {
+#pragma unroll
+              for (int r = 0; r < R; ++r) buf[lay(j + r * P)] = out[u + r * U];
+            }
+          }
+        }
+        lds_sync<Lay::kWave>();
+        const float2* rp = buf + lay(t);
+#pragma unroll
+        for (int v = h * (E / 2); v < (h + 1) * (E / 2); ++v) x[v] = rp[Lay::off(T * v - h * H)];
+        lds_sync<Lay::kWave>();
+      }
+      Next::run(x, lds0, lds1, lay, t, tw_next, nxt);
     } else {
       if constexpr (TOTEM_TW_LATE) nxt.load(tw_next, t);
       lds_sync<Lay::kWave>();
@@ -342,7 +379,8 @@ template <int N_, int T_, int... RS> struct PlanDef {
   static constexpr int T = T_;
   static constexpr int NRAD = sizeof...(RS);
   static constexpr int EXCH = NRAD - 1;  // LDS exchanges per transform
-  template <int S, class L, bool PP, int START> using St = Stages<N, T, S, L, PP, 1, START, RS...>;
+  template <int S, class L, bool PP, int START, bool SPLIT = false>
+  using St = Stages<N, T, S, L, PP, 1, START, SPLIT, RS...>;
   static constexpr int radix(int i) {
     constexpr int r[] = {RS...};
     return r[i];
@@ -372,10 +410,10 @@ template <int N> __host__ __device__ constexpr int twiddle_len() {
 // x holds elements t + T*v of one transform; after the call it holds the
 // corresponding outputs.  START selects the first LDS buffer: a transform that
 // follows another must start on the buffer the previous one did not use last.
-template <int N, int SIGN, bool PP, int START = 0, class Lay>
+template <int N, int SIGN, bool PP, int START = 0, bool SPLIT = false, class Lay>
 __device__ __forceinline__ void block_fft(float2* x, float2* lds0, float2* lds1, const Lay& lay, int t,
                                           const float2* tw) {
-  using St = typename Plan<N>::template St<SIGN, Lay, PP, START>;
+  using St = typename Plan<N>::template St<SIGN, Lay, PP, START, SPLIT>;
   typename St::Tw none;  // the first stage has no twiddles
   St::run(x, lds0, lds1, lay, t, tw, none);
 }

@@ -29,9 +29,10 @@
 #ifndef TOTEM_COLS_MINWAVES
 #define TOTEM_COLS_MINWAVES 1
 #endif
-// Columns per columns-pass workgroup.  The workgroup holds C whole columns in LDS
-// ((N + N/16) * C float2 with the bank padding), which must stay within 64 KiB:
-// 16 columns of 512 points or 4 of 2048 would need 68 KiB.
+// Columns per columns-pass workgroup.  The workgroup exchanges C whole columns
+// through LDS ((N + N/16) * C float2 with the bank padding), which must stay within
+// 64 KiB: 16 columns of 512 points would need 68 KiB, and 4 columns of 2048 points
+// fit only with the exchange split into two rounds of half the buffer.
 #ifndef TOTEM_COLS_C512
 #define TOTEM_COLS_C512 8
 #endif
@@ -42,7 +43,13 @@
 #define TOTEM_COLS_C1536 4
 #endif
 #ifndef TOTEM_COLS_C2048
-#define TOTEM_COLS_C2048 2
+#define TOTEM_COLS_C2048 4
+#endif
+#ifndef TOTEM_COLS_SPLIT2048
+#define TOTEM_COLS_SPLIT2048 1  // 2048-point columns exchange through half buffers
+#endif
+#ifndef TOTEM_ROWS_SPLIT2048
+#define TOTEM_ROWS_SPLIT2048 0  // 2048-point rows: half-buffer exchanges, 4 rows per workgroup
 #endif
 #ifndef TOTEM_ROWS_PP
 #define TOTEM_ROWS_PP 0        // rows pass: ping-pong LDS (1) or one buffer (0)
@@ -70,19 +77,23 @@ namespace totem {
 // launch of 26 probes); at 768 points 128-thread workgroups (4 rows, tiles
 // unchanged) take the rows pass from 100.5 to 96.5 us; 2048 points keep 2 x 8
 // tiles from 128 lanes per row.
+template <int N> constexpr bool rows_split() { return N == 2048 && TOTEM_ROWS_SPLIT2048; }
 template <int N> constexpr int rows_threads() {
-  return (N == 1024 || N == 768) ? TOTEM_ROWS_THREADS_SMALL : TOTEM_ROWS_THREADS;
+  return (N == 1024 || N == 768) ? TOTEM_ROWS_THREADS_SMALL : rows_split<N>() ? 2 * TOTEM_ROWS_THREADS
+                                                                               : TOTEM_ROWS_THREADS;
 }
 template <int N> struct RowCfg {
   static constexpr int T = Plan<N>::T;
   static constexpr int RW = (rows_threads<N>() / T) > 0 ? rows_threads<N>() / T : 1;  // rows per workgroup
   static constexpr int THREADS = RW * T;
-  static constexpr int LL = row_lds_len<N>();
-  static constexpr bool PP = TOTEM_ROWS_PP != 0;
+  static constexpr bool SPLIT = rows_split<N>();
+  static constexpr int LL = SPLIT ? row_lds_len<N / 2>() : row_lds_len<N>();
+  static constexpr bool PP = TOTEM_ROWS_PP != 0 && !SPLIT;
   static constexpr int NBUF = PP ? 2 : 1;
 };
 template <int N> struct ColCfg {
-  static constexpr bool PP = TOTEM_COLS_PP != 0;
+  static constexpr bool SPLIT = N == 2048 && TOTEM_COLS_SPLIT2048;
+  static constexpr bool PP = TOTEM_COLS_PP != 0 && !SPLIT;
   static constexpr int NBUF = PP ? 2 : 1;
   static constexpr int T = Plan<N>::T;
   // columns per workgroup
@@ -92,6 +103,7 @@ template <int N> struct ColCfg {
                            : N == 1536 ? TOTEM_COLS_C1536
                            : N == 2048 ? TOTEM_COLS_C2048 : 8;
   static constexpr int THREADS = T * C;
+  static constexpr int LC = SPLIT ? col_lds_len<N / 2, C>() : col_lds_len<N, C>();  // one exchange buffer
 };
 // Every pass kernel's static LDS (transform buffers plus the detector's wave sums)
 // stays within 64 KiB per workgroup.
@@ -164,7 +176,10 @@ template <int N> __device__ __forceinline__ void row_lane(int tid, int& lr, int&
 // share 128-byte lines) run back to back on one L2, and each L2 only ever sees 1/8
 // of the propagator / a band of the transmission slice.  Any G with G/8 divisible
 // by the units per workgroup gives a bijection; otherwise the plain probe-major
-// order is used.
+// order is used.  Taking one group of every probe of the launch in turn instead
+// (so that consecutive workgroups of an XCD share a group's transmission rows or
+// propagator columns) measured slower at every size (2048 points, 6 probes per
+// launch: rows 91.5 -> 105 us, columns 108 -> 158 us; 1024: columns 88 -> 120 us).
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -188,7 +203,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 #endif
 
 // Unit u (< K) of workgroup `bid` -> (probe item, group g of G per probe).
-template <int K>
+template <int N, int K>
 __device__ __forceinline__ void unit_map(int bid, int u, int G, int& item, int& g) {
 #if TOTEM_XCD_REMAP
   if ((G & 7) == 0 && ((G >> 3) % K) == 0) {
@@ -221,7 +236,7 @@ __device__ __forceinline__ RowUnit row_unit(const ItemDev* __restrict__ items, c
   constexpr int RW = Cfg::RW, G = N / RW;
   RowUnit r;
   int item, rg;
-  unit_map<K>(blockIdx.x, u, G, item, rg);
+  unit_map<N, K>(blockIdx.x, u, G, item, rg);
   const ItemDev it = items[item];
   const JobDev& jb = jobs[it.job];
   r.slice = step - it.start_step;
@@ -278,7 +293,7 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
         }
       }
       if (ua.slice > 0) {
-        block_fft<N, +1, Cfg::PP, 0>(xa, lds, lds1, lay, t, tw);
+        block_fft<N, +1, Cfg::PP, 0, Cfg::SPLIT>(xa, lds, lds1, lay, t, tw);
         const float s = 1.0f / (float)(N * N);
 #pragma unroll
         for (int v = 0; v < E; ++v) xa[v] = cscale(xa[v], s);
@@ -293,7 +308,7 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
           xa[v] = cmul(xa[v], gld(ua.trow + pc));
         }
       }
-      block_fft<N, -1, Cfg::PP, next_start<N, Cfg::PP>(0)>(xa, lds, lds1, lay, t, tw);
+      block_fft<N, -1, Cfg::PP, next_start<N, Cfg::PP>(0), Cfg::SPLIT>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
       for (int v = 0; v < E; ++v) gst(ua.dst + DSTEP * v, xa[v]);
       if (u + 1 < K) __syncthreads();  // LDS reuse by the next unit
@@ -359,7 +374,7 @@ __device__ __forceinline__ ColUnit col_unit(const ItemDev* __restrict__ items, c
   constexpr int G = N / Cfg::C;
   ColUnit r;
   int item;
-  unit_map<K>(blockIdx.x, u, G, item, r.cg);
+  unit_map<N, K>(blockIdx.x, u, G, item, r.cg);
   const ItemDev it = items[item];
   r.jb = &jobs[it.job];
   r.slice = step - it.start_step;
@@ -383,7 +398,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   using Cfg = ColCfg<N>;
   constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C, K = TOTEM_COLS_K;
   constexpr int NW = (Cfg::THREADS + 63) / 64;
-  constexpr int LC = col_lds_len<N, C>();
+  constexpr int LC = Cfg::LC;
   static_assert(Cfg::NBUF * LC * sizeof(float2) + NW * kMaxAper * sizeof(double) <= kLdsLimit,
                 "columns-pass LDS above 64 KiB");
   __shared__ float2 lds[Cfg::NBUF * LC];
@@ -426,7 +441,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
 #pragma unroll
         for (int v = 0; v < E; ++v) hv[v] = bld(hr, hoffa, v * kStep);
       }
-      block_fft<N, -1, Cfg::PP, 0>(xa, lds, lds1, lay, t, tw);
+      block_fft<N, -1, Cfg::PP, 0, Cfg::SPLIT>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
       for (int v = 0; v < E; ++v) xa[v] = cmul(xa[v], TOTEM_COLS_HPREFETCH ? hv[v] : bld(hr, hoffa, v * kStep));
       bool keep = true;
@@ -449,7 +464,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
         }
       }
       if (keep) {
-        block_fft<N, +1, Cfg::PP, next_start<N, Cfg::PP>(0)>(xa, lds, lds1, lay, t, tw);
+        block_fft<N, +1, Cfg::PP, next_start<N, Cfg::PP>(0), Cfg::SPLIT>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
         for (int v = 0; v < E; ++v) bst(ua.wr, voffa, v * kStep, xa[v]);
       }
@@ -506,7 +521,7 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS) k_fft_rows(float2* __restr
   const bool ok = row < rows;
 #pragma unroll
   for (int v = 0; v < E; ++v) x[v] = ok ? w[t + T * v] : make_float2(0.f, 0.f);
-  block_fft<N, SIGN, false, 0>(x, lds, lds, RowLay{lr * Cfg::LL}, t, tw);
+  block_fft<N, SIGN, false, 0, Cfg::SPLIT>(x, lds, lds, RowLay{lr * Cfg::LL}, t, tw);
   if (ok) {
 #pragma unroll
     for (int v = 0; v < E; ++v) w[t + T * v] = x[v];
@@ -520,8 +535,8 @@ k_fft_cols(const float2* __restrict__ src, float2* __restrict__ dst, int epi, fl
            const float2* __restrict__ tw) {
   using Cfg = ColCfg<N>;
   constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C;
-  static_assert(col_lds_len<N, C>() * sizeof(float2) <= kLdsLimit, "column FFT LDS above 64 KiB");
-  __shared__ float2 lds[col_lds_len<N, C>()];
+  static_assert(Cfg::LC * sizeof(float2) <= kLdsLimit, "column FFT LDS above 64 KiB");
+  __shared__ float2 lds[Cfg::LC];
   const int mat = blockIdx.x / G;
   const int cg = blockIdx.x - mat * G;
   const int c = threadIdx.x % C;
@@ -532,7 +547,7 @@ k_fft_cols(const float2* __restrict__ src, float2* __restrict__ dst, int epi, fl
   float2 x[E];
 #pragma unroll
   for (int v = 0; v < E; ++v) x[v] = s[(size_t)(t + T * v) * N];
-  block_fft<N, SIGN, false, 0>(x, lds, lds, ColLay<C>{c}, t, tw);
+  block_fft<N, SIGN, false, 0, Cfg::SPLIT>(x, lds, lds, ColLay<C>{c}, t, tw);
 #pragma unroll
   for (int v = 0; v < E; ++v) {
     float2 y = x[v];
