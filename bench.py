@@ -16,10 +16,8 @@ on a bounded sample of the same workload.
 """
 import argparse
 import json
-import multiprocessing as mp
 import os
 import sys
-import tempfile
 import time
 
 import numpy as np
@@ -30,41 +28,39 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def _cpu_worker_init(path):
-    global _T_HOST
-    _T_HOST = np.load(path, mmap_mode="r")
+def cpu_threads():
+    """Host threads of the CPU baseline: the process's CPU share (OMP_NUM_THREADS where the launcher
+    sets it, as on a one-GPU box), else every core os.cpu_count() reports."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
 
 
-def _cpu_worker(args):
-    from oracle import multislice as oms
-    probe, gfsf, psf, aper_shifted, r0, c0 = args
-    t = np.asarray(_T_HOST, dtype=np.complex128)
-    res, _ = oms.stem_position(t, probe, gfsf, psf, aper_shifted, r0, c0, periodic=True)
-    return res
-
-
-def cpu_baseline(t_dev, inp, positions, cores):
-    """FP64 oracle multislice (numpy pocketfft) over a pool of host processes, one probe per task,
-    like the reference's parfor over scan columns.  Returns (probe.slices/s, wall s, results)."""
-    cfg = inp.cfg
-    t_host = t_dev.cpu().numpy()
-    with tempfile.TemporaryDirectory(dir="/tmp") as d:
-        path = os.path.join(d, "t.npy")
-        np.save(path, t_host)
-        del t_host
-        aper_shifted = np.fft.ifftshift(inp.aper, axes=(-2, -1))
-        tasks = []
-        for dpos in positions:
-            r0 = (cfg.begin_row - 1 + (dpos // cfg.width_red) * cfg.step) % cfg.np_side
-            c0 = (cfg.begin_col - 1 + (dpos % cfg.width_red) * cfg.step) % cfg.np_side
-            tasks.append((inp.probe, inp.gfsf, inp.psf, aper_shifted, r0, c0))
-        ctx = mp.get_context("spawn")
-        with ctx.Pool(cores, initializer=_cpu_worker_init, initargs=(path,)) as pool:
-            pool.map(_cpu_worker, tasks[:1])  # warm the workers (imports, page cache)
-            t0 = time.perf_counter()
-            res = pool.map(_cpu_worker, tasks, chunksize=1)
-            wall = time.perf_counter() - t0
-    return len(positions) * cfg.n_slices / wall, wall, np.asarray(res)
+def cpu_baseline(cfg, atoms, inp, positions, threads):
+    """The reference's multicore CPU path restated in compiled FP64 (oracle/cpu_stem.cpp, OpenMP over
+    slices for the potential and over probe positions for the propagation, like its parfor loops) on
+    the same phonon configuration: the whole potential build from the displaced atoms, then
+    len(positions) probes through every slice.  Returns (potential build s, propagation s, images
+    [len(positions), A])."""
+    from oracle import cpu_stem
+    from totem_amd import stem
+    rows = stem.prepare(cfg, atoms=atoms, optics_too=False)
+    kw = {}
+    if cfg.paraflag == "n":
+        kw = dict(ele_n_corr=rows.ele_n_corr, corr_matrix=rows.corr_matrix, series_n_corr=rows.series_n_corr)
+    ele_i = rows.ele_n_i if len(rows.ele_n_i) else None
+    t0 = time.perf_counter()
+    _, t = cpu_stem.potential_slices(cfg.np_side, rows.ele_n, None, ele_i, None, rows.series_n,
+                                     rows.series_n_i if ele_i is not None else None, inp.s2, inp.gx, inp.gy,
+                                     inp.sigma, inp.parameter, inp.aper_true, cfg.paraflag, threads=threads,
+                                     want_v=False, **kw)
+    build_s = time.perf_counter() - t0
+    ash = np.fft.ifftshift(inp.aper, axes=(-2, -1))
+    r0 = [(cfg.begin_row - 1 + (d // cfg.width_red) * cfg.step) % cfg.np_side for d in positions]
+    c0 = [(cfg.begin_col - 1 + (d % cfg.width_red) * cfg.step) % cfg.np_side for d in positions]
+    t1 = time.perf_counter()
+    img, _ = cpu_stem.stem_positions(t, inp.probe, inp.gfsf, inp.psf, ash, r0, c0, periodic=True, threads=threads)
+    prop_s = time.perf_counter() - t1
+    return build_s, prop_s, img
 
 
 def measured_traffic(kname, n, workload):
@@ -168,31 +164,42 @@ def run_precision_stress(scan):
     return out
 
 
-def run_serving(n_requests, rate, seed, arena_gib, budget_gib):
+def run_serving(n_requests, rate, seed, arena_gib, budget_gb):
     """Poisson arrivals of mixed-size STEM requests (grids 256^2 50%, 512^2 30%, 1024^2 15%,
     2048^2 5%; 10-120 slices; scans 8x8-64x64; Lobato or Peng; SrTiO3 or amorphous C) through the
-    served engine.  Returns the latency percentiles and the aggregate probe.slice/s."""
-    from totem_amd import engine, serve, stem
+    served engine, admitted by footprint against ``budget_gb`` (1e9 bytes).  Each request is
+    prepared on the host (slicing, optics) and built on the GPU after it arrives, so latency
+    (arrival to host images) includes that preparation.  Returns the latency percentiles with the
+    sample count and the aggregate probe.slice/s."""
+    from totem_amd import engine, serve
     rng = np.random.default_rng(seed)
     reqs, t, work = [], 0.0, 0
     mix = {}
     for k in range(n_requests):
         cfg, kind = serve.mixed_request(rng)
         t += rng.exponential(1.0 / rate)
-        reqs.append(serve.Request(rid=k, arrival=t, inp=stem.prepare(cfg), kind=kind))
-        work += cfg.depth * cfg.n_slices
+        reqs.append(serve.Request(rid=k, arrival=t, cfg=cfg, kind=kind))
+        work += serve.focal_count(cfg) * cfg.depth * cfg.n_slices
         mix["%d" % cfg.n] = mix.get("%d" % cfg.n, 0) + 1
     srv = engine.Server(int(arena_gib * 2**30), 8, 16)
     try:
-        wall, peak = serve.serve(reqs, srv, hbm_budget=int(budget_gib * 2**30))
+        wall, peak = serve.serve(reqs, srv, hbm_budget=int(budget_gb * 1e9))
     finally:
         srv.close()
     lat = np.array([r.latency for r in reqs])
-    return {"requests": n_requests, "arrival_rate_per_s": rate, "grids": mix,
+    prep = np.array([r.extra.get("prepare_s", 0.0) for r in reqs])
+    by = {}
+    for r in reqs:
+        by.setdefault("%d" % r.cfg.n, []).append(r.latency)
+    return {"requests": n_requests, "latency_samples": int(len(lat)), "arrival_rate_per_s": rate, "grids": mix,
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-            "max_latency_s": float(lat.max()), "wall_s": wall, "probe_slices": int(work),
-            "throughput_probe_slices_per_s": work / wall, "peak_resident_GiB": peak / 2**30,
-            "arena_GiB": arena_gib, "hbm_budget_GiB": budget_gib, "seed": seed}
+            "max_latency_s": float(lat.max()),
+            "p50_latency_s_by_grid": {k: float(np.percentile(v, 50)) for k, v in sorted(by.items())},
+            "latency_includes": "arrival -> host preparation (slicing, optics) -> transmission build -> "
+                                "served propagation -> images on the host",
+            "p50_host_prepare_s": float(np.percentile(prep, 50)), "wall_s": wall, "probe_slices": int(work),
+            "throughput_probe_slices_per_s": work / wall, "peak_resident_GB": peak / 1e9,
+            "arena_GiB": arena_gib, "hbm_budget_GB": budget_gb, "seed": seed}
 
 
 def run_uneven(n_jobs, seed, arena_gib):
@@ -244,12 +251,14 @@ def main():
     ap.add_argument("--host-slicing", action="store_true",
                     help="displace and slice on the host before the timed loop (default: on the device, per step)")
     ap.add_argument("--plain-probes", type=int, default=16)
-    ap.add_argument("--cpu-probes", type=int, default=16)
-    ap.add_argument("--cpu-cores", type=int, default=16)
+    ap.add_argument("--cpu-probes", type=int, default=0,
+                    help="probe positions of the CPU baseline (0: one per host thread)")
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="host threads of the CPU baseline (0: OMP_NUM_THREADS, else os.cpu_count())")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-jobs", type=int, default=48, help="jobs of the uneven-batch leg (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=24, help="requests of the mixed serving load (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-requests", type=int, default=120, help="requests of the mixed serving load (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--stress-scan", type=int, default=32,
                     help="scan side of the 2048^2 precision-stress leg (0: skip)")
     ap.add_argument("--hrtem-images", type=int, default=256,
@@ -293,7 +302,7 @@ def main():
         return
     if args.only == "serving":
         print(json.dumps({"serving": run_serving(max(args.serve_requests, 1), args.serve_rate, seed=1005,
-                                                 arena_gib=0.25, budget_gib=160)}), flush=True)
+                                                 arena_gib=0.25, budget_gb=288)}), flush=True)
         return
     dev = torch.device("cuda:0")
     if args.workload == "c2":
@@ -446,17 +455,35 @@ def main():
         "roofline_both_passes": {"achieved": both, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": (both / HBM_PEAK_GBS) if both else None},
     }
-    if not args.no_cpu and args.cpu_probes > 0:
-        cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
-        positions = list(range(min(args.cpu_probes, D)))
-        cv, cwall, cres = cpu_baseline(t, inp, positions, cores)
-        gpu = det_ref.T[positions]
+    if not args.no_cpu:
+        threads = args.cpu_cores if args.cpu_cores > 0 else cpu_threads()
+        positions = list(range(min(args.cpu_probes if args.cpu_probes > 0 else threads, D)))
+        if args.host_slicing:
+            atoms = snaps[-1] if sig else cfg.atoms
+        else:
+            atoms = np.array(cfg.atoms, dtype=np.float64)
+            atoms[:, 1:4] = spec.positions(1002, rows[-1])
+            if sig:
+                atoms[:, 6] = 0.0
+        build_cpu, prop_cpu, cres = cpu_baseline(cfg, atoms, inp, positions, threads)
+        # the whole step on the host: one configuration's potential plus D probes, the probes
+        # D / len(positions) times the measured batch of one probe per thread
+        step_cpu = build_cpu + prop_cpu * D * M / (len(positions) * M)
+        g = det_ref.reshape(A, M, D)[:, :, positions]
+        gpu = np.einsum("amd,m->da", g, inp.gfsf)
         rel = float(np.max(np.abs(gpu - cres) / np.abs(cres)))
-        out["cpu_baseline"] = {"value": cv, "unit": "probe.slice/s", "cores": cores, "kind": "port",
-                               "sample": "%d probe positions x %d slices of phonon configuration %d, FP64 numpy "
-                                         "oracle (propagation only, transmission taken from the GPU build), "
-                                         "%.1f s wall" % (len(positions), S, n_cfg - 1, cwall)}
+        out["cpu_baseline"] = {
+            "value": M * D * S / step_cpu, "unit": "probe.slice/s", "cores": threads, "kind": "port",
+            "sample": "phonon configuration %d: potential of all %d slices (%.1f s) and %d probe positions "
+                      "x %d slices (%.1f s), FP64 C++ restatement of the reference CPU path "
+                      "(oracle/cpu_stem.cpp, OpenMP, %d threads; os.cpu_count() %d); value = %d probes x %d "
+                      "slices over the build plus the propagation time scaled to all %d probes"
+                      % (rows[-1] if not args.host_slicing else n_cfg - 1, S, build_cpu, len(positions), S,
+                         prop_cpu, threads, os.cpu_count() or 0, M * D, S, M * D),
+            "potential_build_s": build_cpu,
+            "propagation_probe_slices_per_s": len(positions) * M * S / prop_cpu}
         out["oracle_max_rel_err"] = rel
+        out["oracle_error_covers"] = "GPU potential build + propagation vs the FP64 C++ oracle from the same atoms"
     if args.uneven_jobs > 0 or args.serve_requests > 0 or args.hrtem_images > 0 or args.stress_scan > 0:
         if not args.host_slicing:
             spec.close()
@@ -469,7 +496,7 @@ def main():
     if args.uneven_jobs > 0:
         out["uneven_batch"] = run_uneven(args.uneven_jobs, seed=1004, arena_gib=0.25)
     if args.serve_requests > 0:
-        out["serving"] = run_serving(args.serve_requests, args.serve_rate, seed=1005, arena_gib=0.25, budget_gib=160)
+        out["serving"] = run_serving(args.serve_requests, args.serve_rate, seed=1005, arena_gib=0.25, budget_gb=288)
     print(json.dumps(out))
 
 

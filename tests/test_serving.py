@@ -141,7 +141,7 @@ def test_serving_loop_poisson_mix(engine):
         cfg, kind = serve.mixed_request(rng, grids=((128, 0.4), (256, 0.4), (512, 0.2)), slices=(4, 12),
                                         scan=(2, 6), cells=2)
         t += rng.exponential(0.02)
-        reqs.append(serve.Request(rid=k, arrival=t, inp=stem.prepare(cfg), kind=kind))
+        reqs.append(serve.Request(rid=k, arrival=t, cfg=cfg, kind=kind))
     srv = engine.Server(2 << 30, MAX_APER, MAX_MID)
     try:
         wall, peak = serve.serve(reqs, srv, hbm_budget=8 << 30)

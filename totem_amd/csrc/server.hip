@@ -68,6 +68,9 @@ struct Group {
   std::vector<Cohort> cohorts;
   std::vector<ItemDev> host_items;
   DevBuf items_dev;
+  static constexpr int kAhead = 4;    // slice steps a side may run ahead of its device work
+  hipEvent_t ahead_ev[kAhead] = {};
+  bool ahead_set[kAhead] = {};
   static constexpr int kRing = 3;     // pinned staging buffers of the item list
   ItemDev* stage[kRing] = {nullptr, nullptr, nullptr};
   size_t stage_cap[kRing] = {0, 0, 0};
@@ -118,6 +121,7 @@ Group& group_of(Server& s, int n) {
   g.per_page = (int)(s.page_bytes / g.slot_bytes);
   HIP_OK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
   for (int r = 0; r < Group::kRing; ++r) HIP_OK(hipEventCreateWithFlags(&g.stage_ev[r], hipEventDisableTiming));
+  for (int r = 0; r < Group::kAhead; ++r) HIP_OK(hipEventCreateWithFlags(&g.ahead_ev[r], hipEventDisableTiming));
   tw_table(n);
   return g;
 }
@@ -223,6 +227,10 @@ void upload_items(Server& s, Group& g) {
 
 void step_group(Server& s, Group& g) {
   if (g.cohorts.empty()) return;
+  // at most kAhead steps of this side queued on the device: a job submitted now joins
+  // within a few slice steps instead of behind an unbounded queue
+  const int ring = g.step % Group::kAhead;
+  if (g.ahead_set[ring]) HIP_OK(hipEventSynchronize(g.ahead_ev[ring]));
   if (g.dirty) {
     upload_items(s, g);
     g.dirty = false;
@@ -251,6 +259,8 @@ void step_group(Server& s, Group& g) {
                     g.cohorts.end());
     g.dirty = true;
   }
+  HIP_OK(hipEventRecord(g.ahead_ev[ring], g.st));
+  g.ahead_set[ring] = true;
   ++g.step;
 }
 
@@ -390,6 +400,7 @@ int totem_server_destroy(totem_server* srv) {
         if (g.stage[r]) HIP_OK(hipHostFree(g.stage[r]));
         HIP_OK(hipEventDestroy(g.stage_ev[r]));
       }
+      for (int r = 0; r < totem::srv::Group::kAhead; ++r) HIP_OK(hipEventDestroy(g.ahead_ev[r]));
       HIP_OK(hipStreamDestroy(g.st));
     }
     for (auto& e : s->page_ev) HIP_OK(hipEventDestroy(e));

@@ -2,16 +2,20 @@
 
 A request is one STEM simulation (specimen, optics, scan).  The loop
 (:func:`serve`) admits arrived requests while their device footprint fits the
-HBM budget, builds each admitted request's transmission stack on the GPU,
-submits its probes to the engine's served form (:class:`engine.Server`, which
-advances probes of all in-flight jobs together, one slice per step, and admits
-new probes as wave slots free up), and records per-request latency from arrival
-to results on the host.
+HBM budget and hands them to a builder thread, which prepares each request on
+the host (specimen slicing, optics) and builds its transmission stack on a
+stream of its own; meanwhile the loop keeps stepping the engine's served form
+(:class:`engine.Server`, which advances probes of all in-flight jobs together,
+one slice per step, and admits new probes as wave slots free up) and submits
+each request as soon as its build has finished.  Latency runs from arrival to
+results on the host, host preparation included.
 
 This is the request loop around the reference's one-call-per-simulation
 ``myProjectfun`` (``cuda_source_lobato.cu:1018-1562``): the arithmetic of each
 job is the same as a call of its own.
 """
+import queue
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -55,7 +59,11 @@ class DeviceJob:
                                    c.begin_row - 1, c.begin_col - 1, self.det.data_ptr(),
                                    aper2_ptr=self.aper2.data_ptr(), mid=sorted(c.mid_slices), wrap=wrap,
                                    exit_ptr=None if self.exit is None else self.exit.data_ptr())
-        torch.cuda.current_stream().synchronize()   # inputs ready before another stream reads them
+        # inputs ready before another stream reads them (the copies above ran on the current
+        # stream, the build on `st`)
+        torch.cuda.current_stream().synchronize()
+        if stream is not None:
+            torch.cuda.ExternalStream(stream).synchronize()
 
     @property
     def probe_slices(self):
@@ -78,12 +86,24 @@ class DeviceJob:
         return myresul, mid
 
 
-def footprint_bytes(cfg, M, A):
-    """Device bytes a request keeps resident while in service (transmission stack, probes,
-    propagator, masks, outputs); wave slots come from the server's arena."""
+def footprint_bytes(cfg, M=None, A=None):
+    """Device bytes a request holds while in service: transmission stack, probes, propagator,
+    masks and outputs, plus the potential builder's phase tables for its build (about 16 bytes per
+    term and pixel row of the largest slice, counted with the specimen's rows per slice; the
+    builder releases large tables after a build).  Wave slots come from the server's arena."""
     n2, np2 = cfg.n * cfg.n, cfg.np_side * cfg.np_side
     L = len(cfg.mid_slices)
-    return int(8 * cfg.n_slices * np2 + 8 * M * n2 + 8 * n2 + 16 * A * n2 + 8 * (L + 1) * A * M * cfg.depth)
+    M = focal_count(cfg) if M is None else M
+    A = len(cfg.detectors_mrad) if A is None else A
+    rows = 2 * len(cfg.atoms) / max(cfg.n_slices, 1) + 64     # corrections double the rows
+    build = int(16 * rows * cfg.np_side)
+    return int(8 * cfg.n_slices * np2 + 8 * M * n2 + 8 * n2 + 16 * A * n2 + 8 * (L + 1) * A * M * cfg.depth) + build
+
+
+def focal_count(cfg):
+    """Focal-spread members the probe stack of ``cfg`` will hold (``Gaussian_focal``: one member
+    unless (mtotal - 1) // 2 > 0)."""
+    return cfg.focal_members if (cfg.focal_members - 1) // 2 > 0 else 1
 
 
 # ----------------------------------------------------------------- request mix
@@ -92,9 +112,11 @@ def footprint_bytes(cfg, M, A):
 class Request:
     rid: int
     arrival: float
-    inp: object
-    kind: str
+    cfg: object
+    kind: str = ""
+    inp: object = None          # prepared inputs (prepared by the serving loop's builder when None)
     admitted: float = None
+    built: float = None
     done: float = None
     job: DeviceJob = None
     jid: int = None
@@ -174,47 +196,92 @@ def run_all(server, jobs, max_wall=600.0):
 
 # ----------------------------------------------------------------- serving loop
 
-def serve(requests, server, hbm_budget, build_stream=None, max_wall=600.0, on_done=None):
+def serve(requests, server, hbm_budget, max_wall=600.0, on_done=None):
     """Run the requests (sorted by ``arrival``, seconds after the start) through ``server``.
 
-    A request is admitted when it has arrived and its footprint fits ``hbm_budget`` next to
-    the requests in service (first come, first served).  Returns (wall seconds, peak resident
-    bytes); each request gets ``admitted``, ``done`` (seconds) and ``result`` (mex-shaped images)."""
+    A request is admitted when it has arrived and its footprint fits ``hbm_budget`` next to the
+    requests in service or being built (first come, first served; a request larger than the budget
+    is admitted alone).  A builder thread prepares admitted requests (``stem.prepare`` when
+    ``inp`` is None) and builds their transmission stacks on its own stream; the loop submits each
+    one when its build is complete and keeps stepping the server in between.  Returns (wall
+    seconds, peak resident bytes); each request gets ``admitted``, ``built``, ``done`` (seconds),
+    ``extra['prepare_s']`` and ``result`` (mex-shaped images)."""
     import torch
     waiting = deque(sorted(requests, key=lambda r: r.arrival))
     arrived = deque()
     in_service = {}
-    resident = peak = 0
+    todo, ready = queue.Queue(), queue.Queue()
+    errors = []
+    dev = torch.cuda.current_device()
     t0 = time.perf_counter()
-    while waiting or arrived or in_service:
-        now = time.perf_counter() - t0
-        if now > max_wall:
-            raise RuntimeError("serving loop exceeded %.0f s" % max_wall)
-        while waiting and waiting[0].arrival <= now:
-            arrived.append(waiting.popleft())
-        while arrived:
-            r = arrived[0]
-            fp = footprint_bytes(r.inp.cfg, len(r.inp.gfsf), r.inp.aper.shape[0])
-            if in_service and resident + fp > hbm_budget:
-                break
-            arrived.popleft()
-            r.admitted = time.perf_counter() - t0
-            r.job = DeviceJob(r.inp, stream=build_stream)
-            r.jid = server.submit(r.job.job)
-            in_service[r.jid] = r
-            resident += fp
-            peak = max(peak, resident)
-        if in_service:
-            server.step()
-            for jid in server.poll():
-                r = in_service.pop(jid)
-                r.result = r.job.images()
-                r.done = time.perf_counter() - t0
-                resident -= footprint_bytes(r.inp.cfg, len(r.inp.gfsf), r.inp.aper.shape[0])
-                if on_done:
-                    on_done(r)
-                r.job = None
-        elif waiting:
-            time.sleep(max(0.0, min(0.01, waiting[0].arrival - (time.perf_counter() - t0))))
+
+    def builder():
+        torch.cuda.set_device(dev)
+        st = torch.cuda.Stream(device=dev)
+        while True:
+            r = todo.get()
+            if r is None:
+                return
+            try:
+                tp = time.perf_counter()
+                if r.inp is None:
+                    r.inp = stem.prepare(r.cfg)
+                r.extra["prepare_s"] = time.perf_counter() - tp
+                with torch.cuda.stream(st):
+                    r.job = DeviceJob(r.inp, stream=st.cuda_stream)
+                r.built = time.perf_counter() - t0
+            except BaseException as e:   # reported on the loop thread
+                errors.append(e)
+            ready.put(r)
+
+    th = threading.Thread(target=builder, name="totem-serve-builder", daemon=True)
+    th.start()
+    building = 0
+    resident = peak = 0
+    try:
+        while waiting or arrived or in_service or building:
+            now = time.perf_counter() - t0
+            if now > max_wall:
+                raise RuntimeError("serving loop exceeded %.0f s" % max_wall)
+            while waiting and waiting[0].arrival <= now:
+                arrived.append(waiting.popleft())
+            while arrived:
+                r = arrived[0]
+                fp = footprint_bytes(r.cfg)
+                if (in_service or building) and resident + fp > hbm_budget:
+                    r.extra["budget_wait"] = True
+                    break
+                arrived.popleft()
+                r.admitted = time.perf_counter() - t0
+                resident += fp
+                peak = max(peak, resident)
+                building += 1
+                todo.put(r)
+            while True:
+                try:
+                    r = ready.get_nowait()
+                except queue.Empty:
+                    break
+                building -= 1
+                if errors:
+                    raise errors[0]
+                r.jid = server.submit(r.job.job)
+                in_service[r.jid] = r
+            if in_service:
+                server.step()
+                for jid in server.poll():
+                    r = in_service.pop(jid)
+                    r.result = r.job.images()
+                    r.done = time.perf_counter() - t0
+                    resident -= footprint_bytes(r.cfg)
+                    if on_done:
+                        on_done(r)
+                    r.job = None
+            else:
+                nxt = waiting[0].arrival - (time.perf_counter() - t0) if waiting else 0.001
+                time.sleep(max(0.0, min(0.002, nxt)))
+    finally:
+        todo.put(None)
+        th.join()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, peak
