@@ -3,11 +3,13 @@
 CPU: pins of the FP64 oracle (oracle/hrtem.py) -- no aberrations give the plain image; its chi is
 the MATLAB residual aberration function (optics.residual_chi, restated from another reference
 file); its gradient is (1/lambda) grad W, by central differences, for every term the reference
-writes as a derivative (the v-gradient's A2 slot, which holds A3, and the A5 terms are reproduced as
-written and excluded); the envelope is transposed; the parameter draw; and the golden images on the
-reference's pydata.mat inputs.
+writes as a derivative, and its two deviations are pinned the same way: the A5 terms are the
+gradient of a stated polynomial P5 (not of W's A5 term), and the v-gradient's A2 slot is the A2
+derivative evaluated with A3's coefficients; the envelope is transposed; the parameter draw; and
+the golden images on the reference's pydata.mat inputs.
 GPU: the engine (totem_hrtem_images) against the golden images (<= 1e-5 relative L2 per crop,
-1e-5 on image sums) and the tool's producer/consumer call shape.
+1e-5 on image sums), against the oracle with every aberration drawn, and the tool's
+producer/consumer call shape.
 """
 import os
 import queue
@@ -82,6 +84,53 @@ def test_gradient_is_grad_w_over_lambda(i):
         np.testing.assert_allclose(gv, nv, rtol=1e-8, atol=1e-8 * np.abs(nv).max())
 
 
+def _p5(p, u, v, la):
+    """la^5/6 [A5x (u^4 + v^4 - 14 u v)(u^2 - v^2) - A5y 2 u v (3 u^4 + 3 v^4 - 10 u^2 v^2)]: the
+    polynomial whose gradient the reference writes for A5 (cuda_stem64.py:411, :504-505) in
+    place of that of Re(A5/6 conj(w)^6)."""
+    A5x, A5y = p[43], p[44]
+    return la ** 5 / 6 * (A5x * (u ** 4 + v ** 4 - 14 * u * v) * (u ** 2 - v ** 2)
+                          - A5y * 2 * u * v * (3 * u ** 4 + 3 * v ** 4 - 10 * u ** 2 * v ** 2))
+
+
+@pytest.mark.parametrize("i", [43, 44])
+def test_a5_gradient_terms_are_the_gradient_of_p5(i):
+    u = np.array([[3.1, -7.3], [12.2, 5.5]])
+    v = np.array([[-4.4, 9.1], [2.2, -11.7]])
+    p = np.zeros(49)
+    p[0], p[i] = LAM, 3e6
+    gu, gv = oh.gradient(p, u, v, LAM)
+    h = 1e-5
+    nu = (_p5(p, u + h, v, LAM) - _p5(p, u - h, v, LAM)) / (2 * h)
+    nv = (_p5(p, u, v + h, LAM) - _p5(p, u, v - h, LAM)) / (2 * h)
+    np.testing.assert_allclose(gu, nu, rtol=1e-7, atol=1e-8 * np.abs(nu).max())
+    np.testing.assert_allclose(gv, nv, rtol=1e-7, atol=1e-8 * np.abs(nv).max())
+    # and it is not the gradient of the A5 term of W
+    wu = (oh.aberration_w(p, u + h, v, LAM) - oh.aberration_w(p, u - h, v, LAM)) / (2 * h) / LAM
+    assert np.max(np.abs(gu - wu)) > 0.1 * np.abs(wu).max()
+
+
+def test_v_gradient_reads_a3_in_the_a2_slot():
+    """cuda_stem64.py:461-462: the v-gradient's A2 term is d/dv of Re(A2/3 conj(w)^3) / lambda
+    evaluated with A3's coefficients, so A2 adds nothing to g_v and A3 adds that term too."""
+    u = np.array([[3.1, -7.3], [12.2, 5.5]])
+    v = np.array([[-4.4, 9.1], [2.2, -11.7]])
+    h = 1e-5
+    for i in (28, 29):
+        p = np.zeros(49)
+        p[0], p[i] = LAM, 50.0
+        _, gv = oh.gradient(p, u, v, LAM)
+        assert np.all(gv == 0.0)
+    p = np.zeros(49)
+    p[0], p[32], p[33] = LAM, 700.0, -400.0
+    q = np.zeros(49)
+    q[0], q[28], q[29] = LAM, p[32], p[33]          # W's A2 term with A3's coefficients
+    w = lambda vv: oh.aberration_w(p, u, vv, LAM) + oh.aberration_w(q, u, vv, LAM)
+    nv = (w(v + h) - w(v - h)) / (2 * h) / LAM
+    _, gv = oh.gradient(p, u, v, LAM)
+    np.testing.assert_allclose(gv, nv, rtol=1e-8, atol=1e-8 * np.abs(nv).max())
+
+
 def test_envelope_is_transposed():
     U, V = _grid(32)
     p = np.zeros(49)
@@ -130,6 +179,32 @@ def test_engine_matches_golden_images(engine, name):
         print("  image %d: crop rel L2 %.3e (FP64-oracle error)" % (b, err))
         assert err <= 1e-5
     np.testing.assert_allclose(img.sum(axis=(1, 2)), gold[name + "_sum"], rtol=1e-5)
+
+
+@pytest.mark.gpu
+def test_engine_matches_oracle_with_every_aberration_drawn(engine):
+    """All 21 draw ranges non-zero (A5, D4, B4, S5, C5, D5 included, which the golden inputs do
+    not draw) on a 128^2 grid with a 3-member focal series."""
+    from totem_amd import hrtem
+    print("case hrtem every aberration drawn", flush=True)
+    n, lam, sampling = 128, 0.00417552, 0.02
+    U, V = _grid(n, sampling)
+    rng = np.random.default_rng(11)
+    w = np.exp(1j * rng.uniform(-0.4, 0.4, (n, n)))
+    mywave = np.fft.fftshift(np.fft.fft2(w))
+    pp2 = np.zeros(49)
+    pp2[0], pp2[1], pp2[2], pp2[8] = lam, 10.0, 10.0, 1.0e3
+    mrr = np.abs(rng.normal(0.0, 5.0, 21)) + 1.0
+    mrr[18] = 2e5                                    # A5 magnitude (param 19)
+    pp1 = {"lambda": lam, "alafa": 3e-4, "delta_yita": 2.0, "gfsf": 0.7, "mtotal": 3}
+    ai = rng.random((4, 21))
+    img, paras = hrtem.images(pp1, pp2, mrr, U, V, mywave, ai)
+    assert np.all(np.abs(paras[:, 43:45]).max(axis=1) > 1e3)
+    ref = oh.images(mywave, paras, U, V, lam, pp1["alafa"], 3, pp1["delta_yita"], pp1["gfsf"])
+    for b in range(len(ai)):
+        err = np.linalg.norm(img[b] - ref[b]) / np.linalg.norm(ref[b])
+        print("  image %d: rel L2 %.3e (FP64-oracle error)" % (b, err))
+        assert err <= 1e-5
 
 
 @pytest.mark.gpu

@@ -54,64 +54,120 @@ struct C2 {
   double x, y;
 };
 __device__ __forceinline__ C2 cm(C2 a, C2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-__device__ __forceinline__ double re_mul(double px, double py, C2 a) { return px * a.x - py * a.y; }
 
-// W = Re(A1 w*^2)/2 + C1 |w|^2/2 + chi  (getKai :258-311, getW :314-336)
-__device__ double hrtem_w(const double* p, double u, double v, double la) {
+// The aberration function W of getW (:314-336) with getKai's residual chi (:258-311)
+// as a sum of terms Re(C s^m w^n), w = lambda (u + i v), s = conj(w); C = (p[ix] + i p[iy]) / div
+// (iy < 0: a real coefficient).
+struct AbTerm {
+  int ix, iy;
+  double div;
+  int m, n;
+};
+__constant__ AbTerm kAbTerms[14] = {
+    {26, 27, 2, 2, 0},  // A1   Re(A1 s^2) / 2
+    {1, -1, 2, 1, 1},   // C1   C1 |w|^2 / 2
+    {28, 29, 3, 3, 0},  // A2   Re(A2 s^3) / 3
+    {30, 31, 1, 1, 2},  // B2   Re(B2 w^2 s)
+    {32, 33, 4, 4, 0},  // A3   Re(A3 s^4) / 4
+    {34, 35, 1, 1, 3},  // S3   Re(S3 s w^3)
+    {8, -1, 4, 2, 2},   // C3   C3 |w|^4 / 4
+    {37, 38, 5, 5, 0},  // A4   Re(A4 s^5) / 5
+    {39, 40, 1, 1, 4},  // D4   Re(D4 w^4 s)
+    {41, 42, 1, 2, 3},  // B4   Re(B4 w^3 s^2)
+    {43, 44, 6, 6, 0},  // A5   Re(A5 s^6) / 6
+    {45, 46, 1, 2, 4},  // S5   Re(S5 w^4 s^2)
+    {21, -1, 6, 3, 3},  // C5   C5 |w|^6 / 6
+    {47, 48, 1, 1, 5},  // D5   Re(D5 w^5 s)
+};
+constexpr int kTermA2 = 2, kTermA3 = 4, kTermA5 = 10;
+
+struct AbPowers {
+  C2 sp[7], wp[7];  // s^k, w^k, k = 0..6
+};
+__device__ __forceinline__ AbPowers ab_powers(double u, double v, double la) {
+  AbPowers P;
   const C2 w = {la * u, la * v}, s = {la * u, -la * v};
-  const C2 s2 = cm(s, s), s3 = cm(s2, s), s4 = cm(s2, s2), s5 = cm(s4, s), s6 = cm(s3, s3);
-  const C2 w2 = cm(w, w), w3 = cm(w2, w), w4 = cm(w2, w2), w5 = cm(w4, w);
-  const double r2 = w.x * w.x + w.y * w.y;  // |w|^2
-  double chi = re_mul(p[28] / 3, p[29] / 3, s3) + re_mul(p[30], p[31], cm(w2, s)) +
-               re_mul(p[32] / 4, p[33] / 4, s4) + re_mul(p[34], p[35], cm(s, w3)) + p[8] * (r2 * r2) / 4 +
-               re_mul(p[37] / 5, p[38] / 5, s5) + re_mul(p[39], p[40], cm(w4, s)) +
-               re_mul(p[41], p[42], cm(w3, s2)) + re_mul(p[43] / 6, p[44] / 6, s6) +
-               re_mul(p[45], p[46], cm(w4, s2)) + p[21] * (r2 * r2 * r2) / 6 + re_mul(p[47], p[48], cm(w5, s));
-  return re_mul(p[26], p[27], s2) / 2 + p[1] * r2 / 2 + chi;
+  P.sp[0] = P.wp[0] = {1.0, 0.0};
+#pragma unroll
+  for (int k = 1; k < 7; ++k) {
+    P.sp[k] = cm(P.sp[k - 1], s);
+    P.wp[k] = cm(P.wp[k - 1], w);
+  }
+  return P;
+}
+__device__ __forceinline__ C2 ab_coef(const double* p, const AbTerm& t) {
+  return {p[t.ix] / t.div, t.iy >= 0 ? p[t.iy] / t.div : 0.0};
 }
 
-// gradient terms exactly as the reference writes them (tempgradient_u1/u2 :338-420,
-// tempgradient_v1/v2 :431-512), including its A3 slot in the v-gradient and its A5 terms
+__device__ double hrtem_w(const double* p, double u, double v, double la) {
+  const AbPowers P = ab_powers(u, v, la);
+  double W = 0.0;
+#pragma unroll
+  for (int k = 0; k < 14; ++k) {
+    const AbTerm t = kAbTerms[k];
+    const C2 c = ab_coef(p, t), z = cm(c, cm(P.sp[t.m], P.wp[t.n]));
+    W += z.x;
+  }
+  return W;
+}
+
+// (1/lambda) grad W of one term, by the Wirtinger derivatives of F = C s^m w^n:
+// d/du = lambda (dF/dw + dF/ds), d/dv = i lambda (dF/dw - dF/ds), so
+// (1/lambda) dRe(F)/du = Re(C (n s^m w^(n-1) + m s^(m-1) w^n)) and
+// (1/lambda) dRe(F)/dv = Re(i C (n s^m w^(n-1) - m s^(m-1) w^n)).
+__device__ __forceinline__ void term_grad(C2 c, int m, int n, const AbPowers& P, double& gu, double& gv) {
+  C2 dw = {0.0, 0.0}, ds = {0.0, 0.0};
+  if (n > 0) {
+    dw = cm(P.sp[m], P.wp[n - 1]);
+    dw = {n * dw.x, n * dw.y};
+  }
+  if (m > 0) {
+    ds = cm(P.sp[m - 1], P.wp[n]);
+    ds = {m * ds.x, m * ds.y};
+  }
+  const C2 a = cm(c, {dw.x + ds.x, dw.y + ds.y});
+  const C2 b = cm(c, {dw.x - ds.x, dw.y - ds.y});
+  gu += a.x;
+  gv -= b.y;  // Re(i z) = -Im(z)
+}
+
+// The gradient the reference's kernels compute (tempgradient_u1/u2 :338-420,
+// tempgradient_v1/v2 :431-512): (1/lambda) grad W, except for two terms written
+// differently there and kept here as written:
+//  * the v-gradient's A2 term is d/dv of Re(A2 s^3)/3 evaluated with A3's
+//    coefficients (:461-462), so A2 adds nothing to g_v;
+//  * the A5 terms are the gradient of
+//      P5 = lambda^5/6 [A5x (u^4 + v^4 - 14uv)(u^2 - v^2) - A5y 2uv (3u^4 + 3v^4 - 10u^2 v^2)]
+//    (:411, :504-505) instead of that of Re(A5 s^6)/6.
 __device__ void hrtem_grad(const double* p, double u, double v, double la, double& gu, double& gv) {
-  const double A1x = p[26], A1y = p[27], A2x = p[28], A2y = p[29], B2x = p[30], B2y = p[31];
-  const double A3x = p[32], A3y = p[33], S3x = p[34], S3y = p[35], C3 = p[8];
-  const double A4x = p[37], A4y = p[38], C1 = p[1];
-  const double D4x = p[39], D4y = p[40], B4x = p[41], B4y = p[42], A5x = p[43], A5y = p[44];
-  const double S5x = p[45], S5y = p[46], C5 = p[21], D5x = p[47], D5y = p[48];
-  const double u2 = u * u, v2 = v * v, u3 = u2 * u, v3 = v2 * v, u4 = u2 * u2, v4 = v2 * v2;
-  const double r2 = u2 + v2, la2 = la * la, la3 = la2 * la, la4 = la2 * la2, la5 = la4 * la;
-  gu = la * (A1x * u + A1y * v) + A2x * la2 / 3 * (3 * u2 - 3 * v2) + A2y * la2 / 3 * 6 * u * v +
-       B2x * (3 * u2 + v2) * la2 - B2y * la2 * 2 * u * v +
-       1.0 / 4 * la3 * (A3x * (4 * u3 - 12 * u * v2) + A3y * (12 * u2 * v - 4 * v3)) +
-       la3 * (2 * u * (S3x * (u2 - v2) - S3y * 2 * u * v) + r2 * (2 * S3x * u - 2 * S3y * v)) + C3 * la3 * r2 * u +
-       1.0 / 5 * la4 * (A4x * (5 * u4 + 5 * v4 - 30 * u2 * v2) + A4y * (20 * u3 * v - 20 * u * v3));
-  gu += la4 * ((2 * u) * (D4x * (u3 - 3 * u * v2) - D4y * (3 * u2 * v - v3)) +
-               r2 * (D4x * (3 * u2 - 3 * v2) - D4y * 6 * u * v)) +
-        la4 * (2 * r2 * 2 * u * (B4x * u - B4y * v) + r2 * r2 * B4x) +
-        1.0 / 6 * la5 *
-            (A5x * (4 * u3 - 14 * v) * (u2 - v2) + A5x * (u4 + v4 - 14 * u * v) * 2 * u -
-             A5y * 2 * v * (3 * u4 + 3 * v4 - 10 * u2 * v2) - A5y * 2 * v * u * (12 * u3 - 20 * u * v2)) +
-        la5 * (2 * r2 * 2 * u * (S5x * (u2 - v2) - 2 * u * v * S5y) + r2 * r2 * (2 * S5x * u - 2 * v * S5y)) +
-        1.0 / 6 * la5 * C5 * 3 * r2 * r2 * 2 * u +
-        la5 * (2 * u * (D5x * (u4 + v4 - 6 * u2 * v2) - D5y * 4 * u * v * (u2 - v2)) +
-               r2 * (D5x * (4 * u3 - 12 * u * v2) - D5y * (12 * u2 * v - 4 * v3))) +
-        C1 * la * u;
-  gv = la * (A1y * u - A1x * v) - A3x * la2 / 3 * 6 * u * v + A3y * la2 / 3 * (-3 * v2 + 3 * u2) +
-       B2x * la2 * 2 * u * v - B2y * la2 * (u2 + 3 * v2) +
-       1.0 / 4 * la3 * (A3x * (4 * v3 - 12 * u2 * v) + A3y * (4 * u3 - 12 * u * v2)) +
-       la3 * (2 * v * (S3x * (u2 - v2) - S3y * 2 * u * v) + r2 * (-2 * S3x * v - 2 * S3y * u)) +
-       C3 * la3 * r2 * v + 1.0 / 5 * la4 * (A4x * (20 * u * v3 - 20 * u3 * v) + A4y * (5 * v4 + 5 * u4 - 30 * u2 * v2));
-  gv += la4 * ((2 * v) * (D4x * (u3 - 3 * u * v2) - D4y * (3 * u2 * v - v3)) +
-               r2 * (-D4x * 6 * u * v - D4y * (3 * u2 - 3 * v2))) +
-        la4 * (2 * r2 * 2 * v * (B4x * u - B4y * v) - r2 * r2 * B4y) +
-        1.0 / 6 * la5 *
-            (A5x * (4 * v3 - 14 * u) * (u2 - v2) + A5x * (u4 + v4 - 14 * u * v) * (-2 * v) -
-             A5y * 2 * u * (3 * u4 + 3 * v4 - 10 * u2 * v2) - A5y * 2 * v * u * (12 * v3 - 20 * u2 * v)) +
-        la5 * (2 * r2 * 2 * v * (S5x * (u2 - v2) - 2 * u * v * S5y) + r2 * r2 * (-2 * S5x * v - 2 * u * S5y)) +
-        1.0 / 6 * la5 * C5 * 3 * r2 * r2 * 2 * v +
-        la5 * (2 * v * (D5x * (u4 + v4 - 6 * u2 * v2) - D5y * 4 * u * v * (u2 - v2)) +
-               r2 * (D5x * (4 * v3 - 12 * u2 * v) - D5y * (4 * u3 - 12 * u * v2))) +
-        C1 * la * v;
+  const AbPowers P = ab_powers(u, v, la);
+  gu = gv = 0.0;
+#pragma unroll
+  for (int k = 0; k < 14; ++k) {
+    if (k == kTermA5) continue;
+    const AbTerm t = kAbTerms[k];
+    double tu = 0.0, tv = 0.0;
+    term_grad(ab_coef(p, t), t.m, t.n, P, tu, tv);
+    gu += tu;
+    if (k != kTermA2) gv += tv;
+  }
+  {  // v-gradient's A2 slot with A3's coefficients
+    const AbTerm t2 = kAbTerms[kTermA2], t3 = kAbTerms[kTermA3];
+    const C2 c = {p[t3.ix] / t2.div, p[t3.iy] / t2.div};
+    double tu = 0.0, tv = 0.0;
+    term_grad(c, t2.m, t2.n, P, tu, tv);
+    gv += tv;
+  }
+  {  // grad P5
+    const double A5x = p[43], A5y = p[44];
+    const double u2 = u * u, v2 = v * v, u4 = u2 * u2, v4 = v2 * v2;
+    const double f = u4 + v4 - 14 * u * v, g = u2 - v2, q = 3 * u4 + 3 * v4 - 10 * u2 * v2;
+    const double k5 = la * la * la * la * la / 6;
+    gu += k5 * (A5x * ((4 * u2 * u - 14 * v) * g + f * 2 * u) -
+                A5y * (2 * v * q + 2 * u * v * (12 * u2 * u - 20 * u * v2)));
+    gv += k5 * (A5x * ((4 * v2 * v - 14 * u) * g - f * 2 * v) -
+                A5y * (2 * u * q + 2 * u * v * (12 * v2 * v - 20 * u2 * v)));
+  }
 }
 
 // One thread per (batch element, pixel): out[b][swap(r, c)] = wave[r][c] * tl_0(r, c)
