@@ -247,7 +247,9 @@ def main():
     ap.add_argument("--scan", type=int, default=64, help="scan side of the 1024^2 workload (64: 64x64 probes)")
     ap.add_argument("--batch", type=int, default=0,
                     help="probes in flight per launch (0: measured by the engine's autotuner)")
-    ap.add_argument("--no-autotune", action="store_true", help="use the cache-fit rule instead of measuring")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the cache-fit batch and the current pass plan instead of measuring them")
+    ap.add_argument("--plan", type=int, default=-1, help="pass plan of the grid (-1: default / autotuned)")
     ap.add_argument("--host-slicing", action="store_true",
                     help="displace and slice on the host before the timed loop (default: on the device, per step)")
     ap.add_argument("--plain-probes", type=int, default=16)
@@ -335,9 +337,12 @@ def main():
                           cfg.width_red, cfg.step, cfg.begin_row - 1, cfg.begin_col - 1, det.data_ptr(),
                           aper2_ptr=aper2.data_ptr(), wrap=True)
     tuned = None
+    if args.plan >= 0:
+        engine.set_plan(N, args.plan)
     if args.batch <= 0 and not args.no_autotune:
-        tuned = engine.autotune(N)
+        tuned = engine.autotune(N)        # plan x probes per launch
     batch = min(M * D, args.batch if args.batch > 0 else engine.preferred_batch(N))
+    plan_index, plan_desc = engine.plan(N)
     ws_bytes = engine.stem_workspace_bytes(job, batch)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
@@ -437,6 +442,8 @@ def main():
         "data": "synthetic (seeded specimen; phonon configurations drawn on the device)",
         "config": {"workload": name, "grid": N, "slices": S, "probes_per_step": M * D, "probes_per_launch": batch,
                    "batch_autotuned": tuned is not None,
+                   "plan": {"index": plan_index, "of": engine.plan_count(N), "schedule": plan_desc,
+                            "autotuned": tuned is not None},
                    "phonon_configs_timed": args.steps, "paraflag": cfg.paraflag, "graph": bool(args.graph),
                    "specimen": ("host-sliced rows prepared before the timed loop" if args.host_slicing else
                                 "displaced, sliced and built on the device inside every step")},

@@ -109,10 +109,13 @@ int totem_build_transmission(const totem_potential_desc *desc, void *t_dev, doub
                              void *stream);
 
 /* Host-side plan of the same build, no device work: the form-factor classes, the
- * terms over all slices, and split = 1 when Peng rows were split into one class per
- * Gaussian term (done only when that lowers the accumulation's cost).  Diagnostic;
- * no counterpart in the reference. */
-int totem_potential_plan_stats(const totem_potential_desc *desc, int *classes, long long *terms, int *split);
+ * terms over all slices, split = 1 when Peng rows were split into one class per
+ * Gaussian term (done only when that lowers the accumulation's cost), and
+ * paired = 1 when kx/ky pair up (k[np - i] = -k[i]) so that the accumulation forms
+ * a quarter of the reciprocal-space tiles and mirrors the rest.  Diagnostic; no
+ * counterpart in the reference.  Any output pointer may be NULL. */
+int totem_potential_plan_stats(const totem_potential_desc *desc, int *classes, long long *terms, int *split,
+                               int *paired);
 
 /* --------------------------------------------------------- device STEM job
  * The propagation part of myProjectfun with every array already in HBM. */
@@ -149,12 +152,21 @@ typedef struct totem_stem_job {
  * memory-side cache (the batch myProjectfun uses; no counterpart in the
  * reference, which fills free memory, cuda_source_lobato.cu:1249-1260). */
 int totem_stem_preferred_batch(int n);
-/* Measures probes per launch for n x n grids on the current device (a synthetic job
- * through `slices` <= 64 slices at batch sizes around the cache-fit rule) and makes
- * the fastest the device's preferred batch for n from then on; returns it and its
- * probe.slice/s (may be NULL).  Allocates and synchronises; no counterpart in the
- * reference. */
+/* Measures every plan of n x n grids with probes per launch around the cache-fit
+ * rule on the current device (a synthetic job through `slices` <= 64 slices) and
+ * makes the fastest plan and batch the device's for n from then on; returns the
+ * batch and its probe.slice/s (may be NULL); totem_stem_plan reports the plan.
+ * Allocates and synchronises; no counterpart in the reference. */
 int totem_stem_autotune(int n, int slices, int *best_batch, double *probe_slices_per_s);
+/* Pass schedules ("plans") compiled for n-point grids: each fixes the radix plan of
+ * the rows and columns passes, rows and columns per workgroup and the LDS exchange
+ * form.  totem_stem_autotune measures every plan with the batch sizes and keeps the
+ * fastest; totem_stem_set_plan chooses one explicitly.  A run (or a served grid
+ * side) uses the plan current when it starts.  No counterpart in the reference,
+ * whose FFT schedules are cuFFT's (cuda_source_lobato.cu:1324-1327). */
+int totem_stem_plan_count(int n);  /* 0 if n has no plan */
+int totem_stem_plan(int n, int *plan, char *desc, size_t cap);  /* plan in use on the current device */
+int totem_stem_set_plan(int n, int plan);
 /* Device bytes of workspace totem_stem_run needs for `batch` probes in flight. */
 size_t totem_stem_workspace_bytes(const totem_stem_job *job, int batch);
 /* Runs all M*D probes of the job, `batch` at a time (batch = 1 is the plain

@@ -5,12 +5,17 @@ transmission <= 1e-6 relative L2, annular-detector images <= 1e-4 relative per p
 batched / plain / graph-replayed runs are bit-identical.  FP64-oracle errors are
 printed for every case.
 """
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
 from oracle import multislice as oms
 from oracle import potential as opot
 from totem_amd import optics, specimen, stem
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -113,6 +118,34 @@ def test_potential_matches_oracle(engine, flag, ions, absorption):
         assert len(inp.ele_n_i) > 0
     V, t = _build_gpu(engine, inp, absorp)
     Vo, to = _oracle_potential(inp, absorp)
+    for s in range(cfg.n_slices):
+        ev, et = _rel_l2(V[s], Vo[s]), _rel_l2(t[s], to[s])
+        print("  slice %d: V rel L2 %.3e  t rel L2 %.3e" % (s, ev, et))
+        assert ev <= 1e-6
+        assert et <= 1e-6
+
+
+@pytest.mark.parametrize("flag,np_side,shift", [("p", 320, 0.0), ("n", 256, 0.0), ("p", 128, 0.25),
+                                                 ("n", 128, 0.25)])
+def test_potential_paired_and_unpaired_grids_match_oracle(engine, flag, np_side, shift):
+    """Paired frequencies (k[np - i] = -k[i]): a quarter of the tiles is accumulated and each
+    one's mirrored pixels are written from the same products; per-pixel s2 and AperTrue are read
+    at every pixel, so both are made asymmetric here.  A grid shifted by a fraction of a pixel has
+    no pairs and takes the full accumulation.  Both within 1e-6 of the oracle."""
+    from totem_amd import engine as eng
+    print("case potential flag=%s np=%d shift=%.2f" % (flag, np_side, shift), flush=True)
+    cfg, inp, _ = _potential_case(flag, np_side=np_side)
+    rng = np.random.default_rng(np_side)
+    dk = inp.gx[0, 1] - inp.gx[0, 0]
+    inp.gx = inp.gx + shift * dk
+    inp.s2 = inp.s2 * (1.0 + 0.05 * rng.random(inp.s2.shape))
+    inp.aper_true = (rng.random(inp.s2.shape) > 0.2).astype(np.float64)
+    c = inp.cfg
+    st = eng.potential_plan_stats(c.paraflag, c.np_side, c.n_slices, inp.series_n, inp.ele_n, inp.gx, inp.gy,
+                                  inp.s2, inp.aper_true, inp.sigma, inp.parameter)
+    assert st["paired"] == (shift == 0.0)
+    V, t = _build_gpu(engine, inp)
+    Vo, to = _oracle_potential(inp)
     for s in range(cfg.n_slices):
         ev, et = _rel_l2(V[s], Vo[s]), _rel_l2(t[s], to[s])
         print("  slice %d: V rel L2 %.3e  t rel L2 %.3e" % (s, ev, et))
@@ -267,6 +300,81 @@ def test_batched_plain_and_graph_runs_are_bit_identical(engine):
     # and the device path agrees with the oracle fed the same (FP32-rounded) transmission
     res_o, mid_o = _oracle_stem(inp, t=to.astype(np.complex64).astype(np.complex128))
     _check_images(outs[0][1].T, res_o, "device API exit face")
+
+
+def _plan_runs(engine, n, inp, t_dev, plans):
+    """Exit-face detector images [A, M*D] of the job run with each listed pass plan."""
+    torch = _torch()
+    prev, _ = engine.plan(n)
+    outs = []
+    try:
+        for p in plans:
+            engine.set_plan(n, p)
+            job, det = _device_job(engine, inp, t_dev, wrap=True, begin=(0, 0))
+            engine.stem_run(job, inp.cfg.depth, stream=_stream())
+            torch.cuda.synchronize()
+            outs.append(det.cpu().numpy()[-1])
+    finally:
+        engine.set_plan(n, prev)
+    return outs
+
+
+@pytest.mark.parametrize("n", [256, 512, 768, 1024, 1536, 2048])
+def test_every_pass_plan_matches_oracle(engine, n):
+    """Every compiled schedule of the two passes (radix plans, rows / columns per workgroup,
+    exchange form, when the rows pass loads the transmission) gives the same images within FP32
+    rounding, each within 1e-4 of the FP64 C++ oracle (potential and propagation)."""
+    torch = _torch()
+    count = engine.plan_count(n)
+    print("case pass plans n=%d: %d plans" % (n, count), flush=True)
+    assert count >= 2
+    cfg, inp = _stem_case("l", n=n, np_side=n, S=4, cells=4, scan=(2, 2), step=7, begin=(1, 1))
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    from oracle import cpu_stem
+    threads = max(1, min(16, os.cpu_count() or 1))
+    _, to = cpu_stem.potential_slices(n, inp.ele_n, None, None, None, inp.series_n, None, inp.s2, inp.gx, inp.gy,
+                                      inp.sigma, inp.parameter, inp.aper_true, cfg.paraflag, threads=threads,
+                                      want_v=False)
+    t32 = to.astype(np.complex64)
+    t_dev = torch.tensor(t32, device="cuda")
+    outs = _plan_runs(engine, n, inp, t_dev, range(count))
+    ash = np.fft.ifftshift(inp.aper, axes=(-2, -1))
+    r0 = [(d // cfg.width_red) * cfg.step for d in range(cfg.depth)]
+    c0 = [(d % cfg.width_red) * cfg.step for d in range(cfg.depth)]
+    ref, _ = cpu_stem.stem_positions(t32, inp.probe, inp.gfsf, inp.psf, ash, r0, c0, periodic=True,
+                                     threads=threads)
+    for p, o in enumerate(outs):
+        got = o.T
+        rel = float(np.max(np.abs(got - ref) / np.abs(ref)))
+        same = float(np.max(np.abs(o - outs[0]) / np.abs(outs[0])))
+        print("  plan %d: vs FP64 oracle %.2e, vs plan 0 %.2e" % (p, rel, same))
+        assert rel <= 1e-4
+        assert same <= 1e-5
+
+
+def test_autotuned_plan_is_adopted(engine):
+    """totem_stem_autotune measures plan x batch; the chosen pair is then what runs use."""
+    torch = _torch()
+    n = 768
+    prev, _ = engine.plan(n)
+    try:
+        batch, rate = engine.autotune(n, slices=4)
+        plan, desc = engine.plan(n)
+        print("case autotune n=%d: plan %d (%s), batch %d, %.0f probe.slice/s" % (n, plan, desc, batch, rate),
+              flush=True)
+        assert 0 <= plan < engine.plan_count(n) and desc
+        assert engine.preferred_batch(n) == batch and rate > 0
+        cfg, inp = _stem_case("l", n=n, np_side=n, S=3, cells=4, scan=(2, 2), step=7, begin=(1, 1))
+        _, to = _oracle_potential(inp)
+        t_dev = torch.tensor(to.astype(np.complex64), device="cuda")
+        job, det = _device_job(engine, inp, t_dev, wrap=True, begin=(0, 0))
+        engine.stem_run(job, cfg.depth, stream=_stream())         # the adopted plan, not set here
+        torch.cuda.synchronize()
+        adopted = det.cpu().numpy()[-1]
+        explicit = _plan_runs(engine, n, inp, t_dev, [plan])[0]
+        np.testing.assert_array_equal(adopted, explicit)
+    finally:
+        engine.set_plan(n, prev)
 
 
 def _exit_wave_errors(inp, t_host, ew, det, begin, wrap):

@@ -277,6 +277,21 @@ def test_planner_splits_erf_weighted_rows_into_fewer_classes():
     assert not st0["split"] and st0["classes"] == 3
 
 
+def test_planner_reports_paired_frequencies():
+    # the prepared grids come from integer multiples of 1/(np dx): k[np - i] = -k[i] exactly
+    inp = _planner_case(multiceng=0)
+    c = inp.cfg
+    n = c.np_side
+    assert np.array_equal(inp.gx[0, n - 1:0:-1], -inp.gx[0, 1:])
+    assert _plan(inp)["paired"]
+    # a grid offset by a fraction of a pixel has no pairs: the accumulation covers every tile
+    from totem_amd import engine
+    gx = inp.gx + 0.25 * (inp.gx[0, 1] - inp.gx[0, 0])
+    st = engine.potential_plan_stats(c.paraflag, n, c.n_slices, inp.series_n, inp.ele_n, gx, inp.gy, inp.s2,
+                                     inp.aper_true, inp.sigma, inp.parameter)
+    assert not st["paired"]
+
+
 def test_planner_keeps_rows_whose_gaussians_do_not_repeat():
     # per-atom Debye-Waller B folded into every b_q: splitting would make five classes per row
     inp = _planner_case(per_atom_dw=True)

@@ -383,6 +383,7 @@ struct PotDev {
   std::vector<long> term_off, seg_off;  // host, [S + 1]
   size_t max_terms = 0;
   double sigma = 0.0, parameter = 0.0;
+  bool sym = false;  // paired frequencies: a quarter of the tiles (hermitian_pairs)
 };
 
 // Per slice: phase tables, reciprocal-space accumulation, unnormalised inverse FFT
@@ -401,10 +402,10 @@ void potential_run(const PotDev& p, float2* t_dev, double* v_re, double* v_im, h
   for (int s = 0; s < S; ++s) {
     const int nt = (int)(p.term_off[s + 1] - p.term_off[s]);
     float2* ts = t_dev + (size_t)s * np2;
-    phase_tables(st, p.tx + p.term_off[s], p.ty + p.term_off[s], nt, p.kx1, p.ky1, np, dex.as<float2>(),
+    phase_tables(st, p.tx + p.term_off[s], p.ty + p.term_off[s], nt, p.kx1, p.ky1, np, p.sym, dex.as<float2>(),
                  dey.as<float2>());
     accumulate(st, dex.as<float2>(), dey.as<float2>(), p.wt + p.term_off[s], p.segs + p.seg_off[s],
-               (int)(p.seg_off[s + 1] - p.seg_off[s]), p.classes, p.s2, p.aper, p.corr, p.parameter, np, ts);
+               (int)(p.seg_off[s + 1] - p.seg_off[s]), p.classes, p.s2, p.aper, p.corr, p.parameter, np, p.sym, ts);
     if (want_v) {
       fft2d(np, st, ts, ts, 1, +1, 0, 0.f);
       HIP_OK(hipMemcpyAsync(hv.data(), ts, np2 * sizeof(float2), hipMemcpyDeviceToHost, st));
@@ -506,6 +507,7 @@ static void build_transmission_impl(const totem_potential_desc& d, float2* t_dev
   pd.segs = sc->dsegs.as<PotSeg>();
   pd.sigma = d.sigma;
   pd.parameter = d.parameter;
+  pd.sym = hermitian_pairs(P.kx1.data(), P.ky1.data(), P.np);
   potential_run(pd, t_dev, v_re, v_im, st, sc->dex, sc->dey);
   HIP_OK(hipStreamSynchronize(st));
   // the phase tables grow with the terms of the largest slice; a large set is not
@@ -532,7 +534,7 @@ struct totem_specimen {
   int dev = 0;
   int np = 0, S = 0, Ct = 0, n_atoms = 0, n_terms = 0, n_faces = 0, bits = 1, K = 2, split = 0;
   double origin = 0, dz = 0, lo = 0, hi = 0, sigma = 0, parameter = 0, m = 0;
-  bool has_rms = false;
+  bool has_rms = false, sym = false;
   size_t sort_bytes = 0;
   totem::DevBuf xyz, rms, pos, faces, term_atom, term_cls, term_w, term_mode, sub_base, keys, keys2, vals, vals2,
       wslot, counts, sort_tmp;
@@ -628,6 +630,7 @@ static totem_specimen* specimen_create_impl(const totem_specimen_desc& d) {
   sp->sigma = d.pot.sigma;
   sp->parameter = d.pot.parameter;
   sp->has_rms = d.rms != nullptr;
+  sp->sym = hermitian_pairs(P.kx1.data(), P.ky1.data(), np);
   while ((1L << sp->bits) <= (long)S * Ct) ++sp->bits;
   hipStream_t st = lib_stream();
   const size_t np2 = (size_t)np * np, n2 = (size_t)K * std::max(d.n_terms, 1);
@@ -656,7 +659,7 @@ static totem_specimen* specimen_create_impl(const totem_specimen_desc& d) {
   reserve(sp->counts, (size_t)S * Ct * sizeof(int));
   for (DevBuf* b : {&sp->wslot, &sp->tx, &sp->ty, &sp->wt}) reserve(*b, n2 * sizeof(double));
   reserve(sp->segs, std::max<size_t>((size_t)S * Ct, 1) * sizeof(PotSeg));
-  sp->sort_bytes = sort_pairs_temp_bytes((int)n2, sp->bits);
+  sp->sort_bytes = sort_pairs_temp_bytes((int)n2);
   reserve(sp->sort_tmp, sp->sort_bytes);
   sp->h_counts.resize((size_t)S * Ct);
   HIP_OK(hipStreamSynchronize(st));
@@ -696,9 +699,10 @@ static void specimen_transmission_impl(totem_specimen& sp, uint64_t seed, int co
   ka.counts = sp.counts.as<int>();
   term_keys(st, ka);
   HIP_OK(hipGetLastError());
+  unsigned* sorted_vals = sp.vals.as<unsigned>();
   if (n2 > 0)
-    HIP_OK(sort_pairs(st, sp.sort_tmp.p, sp.sort_bytes, sp.keys.as<unsigned>(), sp.keys2.as<unsigned>(),
-                      sp.vals.as<unsigned>(), sp.vals2.as<unsigned>(), n2, sp.bits));
+    sorted_vals = sort_pairs(st, sp.sort_tmp.p, sp.keys.as<unsigned>(), sp.keys2.as<unsigned>(),
+                             sp.vals.as<unsigned>(), sp.vals2.as<unsigned>(), n2, sp.bits);
   HIP_OK(hipMemcpyAsync(sp.h_counts.data(), sp.counts.p, sp.h_counts.size() * sizeof(int), hipMemcpyDeviceToHost,
                         st));
   HIP_OK(hipStreamSynchronize(st));
@@ -725,7 +729,7 @@ static void specimen_transmission_impl(totem_specimen& sp, uint64_t seed, int co
   if (!sp.h_segs.empty())
     HIP_OK(hipMemcpyAsync(sp.segs.p, sp.h_segs.data(), sp.h_segs.size() * sizeof(PotSeg), hipMemcpyHostToDevice,
                           st));
-  term_gather(st, (int)total, sp.K, sp.vals2.as<unsigned>(), sp.term_atom.as<int>(), sp.wslot.as<double>(),
+  term_gather(st, (int)total, sp.K, sorted_vals, sp.term_atom.as<int>(), sp.wslot.as<double>(),
               sp.pos.as<double>(), sp.tx.as<double>(), sp.ty.as<double>(), sp.wt.as<double>());
   HIP_OK(hipGetLastError());
   pd.np = sp.np;
@@ -742,6 +746,7 @@ static void specimen_transmission_impl(totem_specimen& sp, uint64_t seed, int co
   pd.segs = sp.segs.as<PotSeg>();
   pd.sigma = sp.sigma;
   pd.parameter = sp.parameter;
+  pd.sym = sp.sym;
   potential_run(pd, t_dev, nullptr, nullptr, st, sp.ex, sp.ey);
   HIP_OK(hipStreamSynchronize(st));
 }
@@ -761,7 +766,7 @@ struct StemWorkspace {
 // partials per slot, the work items of every probe of the job, and `batch` more
 // items at a fixed address for graph replays.
 static size_t stem_ws_bytes(int n, int S, int A, int L, int batch, long total) {
-  const int G = detector_groups(n);
+  const int G = detector_groups_max(n);
   size_t b = 0;
   b += align_up((size_t)batch * n * n * sizeof(float2));
   b += align_up((size_t)batch * (L + 1) * A * G * sizeof(double));
@@ -773,7 +778,7 @@ static size_t stem_ws_bytes(int n, int S, int A, int L, int batch, long total) {
 }
 
 static StemWorkspace carve(void* base, int n, int S, int A, int L, int batch, long total) {
-  const int G = detector_groups(n);
+  const int G = detector_groups_max(n);
   char* p = static_cast<char*>(base);
   StemWorkspace w;
   w.psi = reinterpret_cast<float2*>(p);
@@ -845,9 +850,11 @@ void window_origin(const totem_stem_job& j, int d, int& r0, int& c0) {
 constexpr size_t kCacheWaveBytes = 208ull << 20;
 static int cache_fit_batch(int n) { return (int)std::max<size_t>(1, kCacheWaveBytes / ((size_t)n * n * sizeof(float2))); }
 
-// Batches measured by totem_stem_autotune, per (device, n); they replace the rule.
+// Batches measured by totem_stem_autotune and pass variants (plans) chosen by it or
+// set by totem_stem_set_plan, per (device, n); they replace the cache-fit rule and
+// variant 0.
 static std::mutex g_tune_mu;
-static std::map<std::pair<int, int>, int> g_tuned;
+static std::map<std::pair<int, int>, int> g_tuned, g_plan;
 
 int preferred_batch(int n) {
   int dev = 0;
@@ -857,6 +864,27 @@ int preferred_batch(int n) {
     if (it != g_tuned.end()) return it->second;
   }
   return cache_fit_batch(n);
+}
+
+int current_plan(int n) {
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_plan.find({dev, n});
+    if (it != g_plan.end()) return it->second;
+  }
+  return 0;
+}
+
+static void set_plan(int n, int plan) {
+  require(size_supported(n), "grid " + std::to_string(n) + " has no compiled FFT plan", TOTEM_ERR_UNSUPPORTED);
+  require(plan >= 0 && plan < pass_variant_count(n),
+          "plan " + std::to_string(plan) + " out of range for n=" + std::to_string(n) + " (" +
+              std::to_string(pass_variant_count(n)) + " compiled)");
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_plan[{dev, n}] = plan;
 }
 
 // Per-kernel durations of the last timed run (TOTEM_RUN_TIMING), from HIP events
@@ -882,7 +910,8 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
     require(ws_bytes >= need, "workspace too small: need " + std::to_string(need) + " bytes", TOTEM_ERR_MEMORY);
   }
   StemWorkspace w = carve(ws, n, S, A, L, batch, total);
-  const int G = detector_groups(n);
+  const int plan = current_plan(n);            // one pass variant for the whole run
+  const int G = detector_groups(n, plan), Gslot = detector_groups_max(n);
   tw_table(n);
   std::vector<int> mos, hit;
   mid_walk(j, mos, hit);
@@ -918,7 +947,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
     window_origin(j, d, it.r0, it.c0);
     it.start_step = 0;
     it.wave = w.psi + (size_t)k * n * n;
-    it.part = w.partial + (size_t)k * (L + 1) * A * G;
+    it.part = w.partial + (size_t)k * (L + 1) * A * Gslot;
     it.out = (int)o;
   }
   ItemDev* const stage = w.items + total;  // graph replays read the chunk's items here
@@ -926,8 +955,8 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   HIP_OK(hipStreamSynchronize(st));  // the host list is a local
   auto record = [&](const ItemDev* it, int cnt) {
     for (int s = 0; s < S; ++s) {
-      pass_rows(n, st, it, cnt, w.job, s);
-      pass_cols(n, st, it, cnt, w.job, s);
+      pass_rows(n, plan, st, it, cnt, w.job, s);
+      pass_cols(n, plan, st, it, cnt, w.job, s);
     }
     reduce_det(st, it, nullptr, cnt, w.job, G, L + 1, A);
   };
@@ -944,9 +973,9 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
     if (flags & TOTEM_RUN_TIMING) {
       HIP_OK(hipEventRecord(ev[0], st));
       for (int s = 0; s < S; ++s) {
-        pass_rows(n, st, ci, cnt, w.job, s);
+        pass_rows(n, plan, st, ci, cnt, w.job, s);
         HIP_OK(hipEventRecord(ev[2 * s + 1], st));
-        pass_cols(n, st, ci, cnt, w.job, s);
+        pass_cols(n, plan, st, ci, cnt, w.job, s);
         HIP_OK(hipEventRecord(ev[2 * s + 2], st));
       }
       reduce_det(st, ci, nullptr, cnt, w.job, G, L + 1, A);
@@ -1029,9 +1058,10 @@ static void run_stem_static_impl(const totem_stem_job& j, int batch, hipStream_t
   HIP_OK(hipStreamSynchronize(st));
 }
 
-// Probes per launch by measurement: a synthetic job (unit transmission and probe,
-// one detector) of 2 x b probes through `slices` slices for b around the cache-fit
-// rule; the fastest per probe.slice is kept for (device, n).
+// Pass variant and probes per launch by measurement: a synthetic job (unit
+// transmission and probe, one detector) of 2 x b probes through `slices` slices,
+// for every compiled variant of n and b around the cache-fit rule; the fastest
+// per probe.slice becomes the (device, n) plan and batch.
 static int autotune_impl(int n, int slices, double* best_rate) {
   require(size_supported(n), "grid " + std::to_string(n) + " has no compiled FFT plan", TOTEM_ERR_UNSUPPORTED);
   require(slices > 0 && slices <= 64, "autotune slices must be in 1..64");
@@ -1068,28 +1098,41 @@ static int autotune_impl(int n, int slices, double* best_rate) {
   hipEvent_t e0, e1;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
-  int best = base;
+  const int prev_plan = current_plan(n);
+  int best = base, best_plan = 0;
   double best_r = 0.0;
-  for (int b : cand) {
-    j.depth = 2 * b;
-    const size_t need = stem_ws_bytes(n, slices, 1, 0, b, 2 * b);
-    if (ws.bytes < need) ws.alloc(need);
-    run_stem_impl(j, b, 0, ws.p, ws.bytes, st);  // warm-up
-    HIP_OK(hipEventRecord(e0, st));
-    run_stem_impl(j, b, 0, ws.p, ws.bytes, st);
-    HIP_OK(hipEventRecord(e1, st));
-    HIP_OK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-    const double rate = 2.0 * b * slices / (ms * 1e-3);
-    if (rate > best_r) {
-      best_r = rate;
-      best = b;
+  try {
+    for (int plan = 0; plan < pass_variant_count(n); ++plan) {
+      set_plan(n, plan);
+      for (int b : cand) {
+        j.depth = 2 * b;
+        const size_t need = stem_ws_bytes(n, slices, 1, 0, b, 2 * b);
+        if (ws.bytes < need) ws.alloc(need);
+        run_stem_impl(j, b, 0, ws.p, ws.bytes, st);  // warm-up
+        HIP_OK(hipEventRecord(e0, st));
+        run_stem_impl(j, b, 0, ws.p, ws.bytes, st);
+        HIP_OK(hipEventRecord(e1, st));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        const double rate = 2.0 * b * slices / (ms * 1e-3);
+        if (rate > best_r) {
+          best_r = rate;
+          best = b;
+          best_plan = plan;
+        }
+      }
     }
+  } catch (...) {
+    set_plan(n, prev_plan);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
   }
   HIP_OK(hipEventDestroy(e0));
   HIP_OK(hipEventDestroy(e1));
   HIP_OK(hipStreamSynchronize(st));
+  set_plan(n, best_plan);
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
   {
@@ -1310,7 +1353,8 @@ int totem_build_transmission(const totem_potential_desc* desc, void* t_dev, doub
   });
 }
 
-int totem_potential_plan_stats(const totem_potential_desc* desc, int* classes, long long* terms, int* split) {
+int totem_potential_plan_stats(const totem_potential_desc* desc, int* classes, long long* terms, int* split,
+                               int* paired) {
   return guarded([&] {
     require(desc != nullptr, "desc is required");
     PotentialPlan P = plan_potential(*desc);
@@ -1319,6 +1363,7 @@ int totem_potential_plan_stats(const totem_potential_desc* desc, int* classes, l
     if (classes) *classes = (int)P.classes.size();
     if (terms) *terms = n;
     if (split) *split = P.split ? 1 : 0;
+    if (paired) *paired = totem::hermitian_pairs(P.kx1.data(), P.ky1.data(), P.np) ? 1 : 0;
   });
 }
 
@@ -1394,6 +1439,24 @@ int totem_stem_autotune(int n, int slices, int* best_batch, double* probe_slices
     require(best_batch != nullptr, "best_batch is required");
     *best_batch = totem::autotune_impl(n, slices, probe_slices_per_s);
   });
+}
+
+int totem_stem_plan_count(int n) { return n > 0 ? totem::pass_variant_count(n) : 0; }
+
+int totem_stem_plan(int n, int* plan, char* desc, size_t cap) {
+  return guarded([&] {
+    require(size_supported(n), "grid " + std::to_string(n) + " has no compiled FFT plan", TOTEM_ERR_UNSUPPORTED);
+    const int p = totem::current_plan(n);
+    if (plan) *plan = p;
+    if (desc && cap > 0) {
+      const std::string d = totem::pass_variant_desc(n, p);
+      std::snprintf(desc, cap, "%s", d.c_str());
+    }
+  });
+}
+
+int totem_stem_set_plan(int n, int plan) {
+  return guarded([&] { totem::set_plan(n, plan); });
 }
 
 size_t totem_stem_workspace_bytes(const totem_stem_job* job, int batch) {

@@ -396,29 +396,36 @@ template <> struct Plan<768> : PlanDef<768, 32, 8, 3, 8, 4> {};
 template <> struct Plan<1024> : PlanDef<1024, 64, 16, 16, 4> {};
 template <> struct Plan<1536> : PlanDef<1536, 64, 8, 3, 8, 8> {};
 template <> struct Plan<2048> : PlanDef<2048, 128, 16, 16, 8> {};
+// Alternative plans a pass variant may use instead of Plan<N>.
+template <int N> struct PlanB;
+template <> struct PlanB<768> : PlanDef<768, 64, 4, 4, 4, 4, 3> {};  // 12 elements per lane
 
 // Number of twiddle entries of a plan's table.
-template <int N> __host__ __device__ constexpr int twiddle_len() {
+template <class PL> __host__ __device__ constexpr int plan_twiddle_len() {
   int p = 1, len = 0;
-  for (int i = 0; i < Plan<N>::NRAD; ++i) {
-    if (p > 1) len += p * Plan<N>::radix(i);
-    p *= Plan<N>::radix(i);
+  for (int i = 0; i < PL::NRAD; ++i) {
+    if (p > 1) len += p * PL::radix(i);
+    p *= PL::radix(i);
   }
   return len;
 }
+template <int N> __host__ __device__ constexpr int twiddle_len() { return plan_twiddle_len<Plan<N>>(); }
 
 // x holds elements t + T*v of one transform; after the call it holds the
 // corresponding outputs.  START selects the first LDS buffer: a transform that
 // follows another must start on the buffer the previous one did not use last.
-template <int N, int SIGN, bool PP, int START = 0, bool SPLIT = false, class Lay>
+template <int N, int SIGN, bool PP, int START = 0, bool SPLIT = false, class PL = Plan<N>, class Lay>
 __device__ __forceinline__ void block_fft(float2* x, float2* lds0, float2* lds1, const Lay& lay, int t,
                                           const float2* tw) {
-  using St = typename Plan<N>::template St<SIGN, Lay, PP, START, SPLIT>;
+  static_assert(PL::N == N, "plan length");
+  using St = typename PL::template St<SIGN, Lay, PP, START, SPLIT>;
   typename St::Tw none;  // the first stage has no twiddles
   St::run(x, lds0, lds1, lay, t, tw, none);
 }
 // START for a transform that follows one started on buffer `prev`.
-template <int N, bool PP> constexpr int next_start(int prev) { return PP ? (prev + Plan<N>::EXCH) % 2 : 0; }
+template <int N, bool PP, class PL = Plan<N>> constexpr int next_start(int prev) {
+  return PP ? (prev + PL::EXCH) % 2 : 0;
+}
 
 // LDS layouts.  Row transforms: one padding slot per 16 elements so that the
 // first stage's stride-R writes spread over the banks.  Column transforms:

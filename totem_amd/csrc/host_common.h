@@ -79,6 +79,8 @@ void validate_job(const totem_stem_job& j);
 void mid_walk(const totem_stem_job& j, std::vector<int>& mos, std::vector<int>& hit);
 // Probes in flight whose waves fit the memory-side cache (the default batch).
 int preferred_batch(int n);
+// Pass variant (plan) of n-point grids in use on the current device (0 unless set).
+int current_plan(int n);
 // The library's own stream on the current device (used when the caller passes none).
 hipStream_t lib_stream();
 // Batch HRTEM imaging, transfer-function imaging and weighted patterns (hrtem.hip).

@@ -21,6 +21,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 // Tuning knobs (compile-time).
 #ifndef TOTEM_ROWS_MINWAVES
@@ -65,6 +66,9 @@ namespace totem {
 
 // ------------------------------------------------------------------------- helpers
 
+#ifndef TOTEM_ROWS_TPREFETCH
+#define TOTEM_ROWS_TPREFETCH 1  // load the transmission window together with the wave
+#endif
 #ifndef TOTEM_ROWS_THREADS
 #define TOTEM_ROWS_THREADS 256  // threads per rows-pass workgroup
 #endif
@@ -77,31 +81,66 @@ namespace totem {
 // launch of 26 probes); at 768 points 128-thread workgroups (4 rows, tiles
 // unchanged) take the rows pass from 100.5 to 96.5 us; 2048 points keep 2 x 8
 // tiles from 128 lanes per row.
-template <int N> constexpr bool rows_split() { return N == 2048 && TOTEM_ROWS_SPLIT2048; }
-template <int N> constexpr int rows_threads() {
-  return (N == 1024 || N == 768) ? TOTEM_ROWS_THREADS_SMALL : rows_split<N>() ? 2 * TOTEM_ROWS_THREADS
-                                                                               : TOTEM_ROWS_THREADS;
-}
-template <int N> struct RowCfg {
-  static constexpr int T = Plan<N>::T;
-  static constexpr int RW = (rows_threads<N>() / T) > 0 ? rows_threads<N>() / T : 1;  // rows per workgroup
+// Pass variants ("plans"): each grid side has one or more schedules for its two
+// passes -- the radix plan of each pass, rows per rows workgroup (which also sets
+// the wave tile shared by both passes), columns per columns workgroup and whether
+// the column exchanges run through half buffers.  Variant 0 is the default; the
+// others are chosen at run time per device (totem_stem_set_plan, or measured by
+// totem_stem_autotune), for a whole run or served grid side at a time.
+template <int N, int V> struct PassSpec {
+  using RowPlan = Plan<N>;
+  using ColPlan = Plan<N>;
+  static constexpr int rows_threads = (N == 1024 || N == 768) ? TOTEM_ROWS_THREADS_SMALL
+                                      : (N == 2048 && TOTEM_ROWS_SPLIT2048) ? 2 * TOTEM_ROWS_THREADS
+                                                                             : TOTEM_ROWS_THREADS;
+  static constexpr bool rows_split = N == 2048 && TOTEM_ROWS_SPLIT2048;
+  static constexpr int cols_c = N < 512 ? 16
+                                : N == 512 ? TOTEM_COLS_C512
+                                : N == 1024 ? TOTEM_COLS_C1024
+                                : N == 1536 ? TOTEM_COLS_C1536
+                                : N == 2048 ? TOTEM_COLS_C2048 : 8;
+  static constexpr bool cols_split = N == 2048 && TOTEM_COLS_SPLIT2048;
+  static constexpr bool rows_tprefetch = TOTEM_ROWS_TPREFETCH;  // transmission loaded with the wave
+};
+// Variant 1 of most sizes: the rows pass loads its transmission window after the
+// inverse transform, not with the wave (fewer registers live across the
+// transform: 1024 points 118 -> 88 VGPRs, the load latency then hidden by more waves).
+template <int N> struct PassSpecLateT : PassSpec<N, 0> {
+  static constexpr bool rows_tprefetch = false;
+};
+template <> struct PassSpec<256, 1> : PassSpecLateT<256> {};
+template <> struct PassSpec<512, 1> : PassSpecLateT<512> {};
+template <> struct PassSpec<1024, 1> : PassSpecLateT<1024> {};
+template <> struct PassSpec<1536, 1> : PassSpecLateT<1536> {};
+template <> struct PassSpec<2048, 1> : PassSpecLateT<2048> {};
+// 768: rows transformed by the 12-element plan 4.4.4.4.3 on 64 lanes (one row per
+// wave, 2 x 8 tiles); columns keep the 24-element plan.
+template <> struct PassSpec<768, 1> : PassSpec<768, 0> {
+  using RowPlan = PlanB<768>;
+  static constexpr int rows_threads = 128;
+};
+template <int N> constexpr int pass_variants() { return (N >= 256 && N != 320) ? 2 : 1; }
+
+template <int N, int V = 0> struct RowCfg {
+  using Spec = PassSpec<N, V>;
+  using PL = typename Spec::RowPlan;
+  static constexpr bool TPREFETCH = Spec::rows_tprefetch;
+  static constexpr int T = PL::T;
+  static constexpr int RW = (Spec::rows_threads / T) > 0 ? Spec::rows_threads / T : 1;  // rows per workgroup
   static constexpr int THREADS = RW * T;
-  static constexpr bool SPLIT = rows_split<N>();
+  static constexpr bool SPLIT = Spec::rows_split;
   static constexpr int LL = SPLIT ? row_lds_len<N / 2>() : row_lds_len<N>();
   static constexpr bool PP = TOTEM_ROWS_PP != 0 && !SPLIT;
   static constexpr int NBUF = PP ? 2 : 1;
 };
-template <int N> struct ColCfg {
-  static constexpr bool SPLIT = N == 2048 && TOTEM_COLS_SPLIT2048;
+template <int N, int V = 0> struct ColCfg {
+  using Spec = PassSpec<N, V>;
+  using PL = typename Spec::ColPlan;
+  static constexpr bool SPLIT = Spec::cols_split;
   static constexpr bool PP = TOTEM_COLS_PP != 0 && !SPLIT;
   static constexpr int NBUF = PP ? 2 : 1;
-  static constexpr int T = Plan<N>::T;
-  // columns per workgroup
-  static constexpr int C = N < 512 ? 16
-                           : N == 512 ? TOTEM_COLS_C512
-                           : N == 1024 ? TOTEM_COLS_C1024
-                           : N == 1536 ? TOTEM_COLS_C1536
-                           : N == 2048 ? TOTEM_COLS_C2048 : 8;
+  static constexpr int T = PL::T;
+  static constexpr int C = Spec::cols_c;  // columns per workgroup
   static constexpr int THREADS = T * C;
   static constexpr int LC = SPLIT ? col_lds_len<N / 2, C>() : col_lds_len<N, C>();  // one exchange buffer
 };
@@ -123,11 +162,13 @@ constexpr int kLdsLimit = TOTEM_LDS_LIMIT;
 // wave (C >= TC columns x 64/C consecutive rows) touches whole lines instead of
 // C x 8-byte segments of 64/C rows.  Element (r, c) sits at
 //   ((r / TR) * (N / TC) + c / TC) * 16 + (r % TR) * TC + c % TC.
-template <int N> struct WaveLay {
-  static constexpr int TR = TOTEM_TILED ? (RowCfg<N>::RW < 4 ? RowCfg<N>::RW : 4) : 1;
+template <int N, int V = 0> struct WaveLay {
+  static constexpr int RW = RowCfg<N, V>::RW, RT = RowCfg<N, V>::T, CT = ColCfg<N, V>::T;
+  static constexpr int TR = TOTEM_TILED ? (RW < 4 ? RW : 4) : 1;
   static constexpr int TC = TOTEM_TILED ? 16 / TR : 1;
-  static_assert(!TOTEM_TILED || (TR * TC == 16 && N % TC == 0 && Plan<N>::T % TC == 0 &&
-                                 Plan<N>::T % TR == 0 && RowCfg<N>::RW % TR == 0), "tile shape");
+  static_assert(!TOTEM_TILED || (TR * TC == 16 && N % TC == 0 && RT % TC == 0 && RT % TR == 0 && RW % TR == 0 &&
+                                 CT % TR == 0),
+                "tile shape");
   __device__ static int at(int r, int c) {
     if constexpr (!TOTEM_TILED) return r * N + c;
     return ((r / TR) * (N / TC) + c / TC) * 16 + (r % TR) * TC + c % TC;
@@ -155,16 +196,24 @@ template <int N> struct WaveLay {
 // barriers -> wave-local row-major -> lane-tiled): 256: 94 -> 88 -> 95,
 // 512: 103.5 -> 100 -> 104, 768: 177 -> 100 -> 133, 1024: 110.5 -> 113 -> 107,
 // 1536: 185 -> 116 -> 137, 2048 (T = 128, no wave-local form): 111 -> - -> 97.
-template <int N> constexpr bool rows_lane_tiled() {
+template <int N, int V = 0> constexpr bool rows_lane_tiled() {
   return TOTEM_ROWS_LANE_TILED == 1 ||
-         (TOTEM_ROWS_LANE_TILED == 2 && !(TOTEM_ROWS_WAVESYNC && Plan<N>::T <= 64 && N != 1024));
+         (TOTEM_ROWS_LANE_TILED == 2 && !(TOTEM_ROWS_WAVESYNC && RowCfg<N, V>::T <= 64 && N != 1024));
 }
-template <int N> constexpr int rows_lds_pad() { return rows_lane_tiled<N>() ? 4 : 0; }
-template <int N> constexpr bool rows_wave_local() {
-  return TOTEM_ROWS_WAVESYNC && !rows_lane_tiled<N>() && Plan<N>::T <= 64 && 64 % Plan<N>::T == 0;
+// LDS row pad of the lane-tiled form: the TR rows of a wave start 16 / TR float2
+// apart modulo 16, so one instruction's accesses of the TR rows fall on disjoint banks.
+#ifndef TOTEM_ROWS_PAD
+#define TOTEM_ROWS_PAD 0  // 0: 16 / TR
+#endif
+template <int N, int V = 0> constexpr int rows_lds_pad();
+template <int N, int V = 0> constexpr bool rows_wave_local() {
+  return TOTEM_ROWS_WAVESYNC && !rows_lane_tiled<N, V>() && RowCfg<N, V>::T <= 64 && 64 % RowCfg<N, V>::T == 0;
 }
-template <int N> __device__ __forceinline__ void row_lane(int tid, int& lr, int& t) {
-  constexpr int T = RowCfg<N>::T, TR = rows_lane_tiled<N>() ? WaveLay<N>::TR : 1;
+template <int N, int V> constexpr int rows_lds_pad() {
+  return !rows_lane_tiled<N, V>() ? 0 : TOTEM_ROWS_PAD ? TOTEM_ROWS_PAD : 16 / WaveLay<N, V>::TR;
+}
+template <int N, int V> __device__ __forceinline__ void row_lane(int tid, int& lr, int& t) {
+  constexpr int T = RowCfg<N, V>::T, TR = rows_lane_tiled<N, V>() ? WaveLay<N, V>::TR : 1;
   const int q = tid / (TR * T), rem = tid - q * (TR * T);
   lr = q * TR + rem % TR;
   t = rem / TR;
@@ -189,9 +238,6 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // ------------------------------------------------------------------------ pass R
 
-#ifndef TOTEM_ROWS_TPREFETCH
-#define TOTEM_ROWS_TPREFETCH 1  // load the transmission window together with the wave
-#endif
 #ifndef TOTEM_ROWS_K
 #define TOTEM_ROWS_K 1  // row groups per workgroup (loads of the next one overlap the FFTs of this one)
 #endif
@@ -219,20 +265,26 @@ __device__ __forceinline__ void unit_map(int bid, int u, int G, int& item, int& 
   g = L - item * G;
 }
 
+// The wave source (the probe, row-major, at slice 0; the item's tiled wave after)
+// and destination are buffer resources: lanes hold 32-bit byte offsets and the
+// offsets between a lane's elements are uniform (scalar registers), so the wave
+// accesses need no 64-bit address arithmetic per element.
 struct RowUnit {
   bool active;
   int slice;
-  const float2* src;   // wave read: probe (slice 0, row-major) or the item's wave
-  int sstep;           // element stride of src between consecutive v
-  float2* dst;         // the item's wave (WaveLay)
-  const float2* trow;  // transmission row
+  __amdgpu_buffer_rsrc_t src, dst;
+  int src_off, src_step;  // bytes: the lane's first element, stride between consecutive v
+  int dst_off;
+  __amdgpu_buffer_rsrc_t trs;  // the transmission slice
+  int t_off;                   // bytes: the lane's row in it
   int c0, np;
 };
 
-template <int N, int K>
+template <int N, int V, int K>
 __device__ __forceinline__ RowUnit row_unit(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs,
                                             int step, int u, int lr, int t) {
-  using Cfg = RowCfg<N>;
+  using Cfg = RowCfg<N, V>;
+  using Lay = WaveLay<N, V>;
   constexpr int RW = Cfg::RW, G = N / RW;
   RowUnit r;
   int item, rg;
@@ -242,75 +294,82 @@ __device__ __forceinline__ RowUnit row_unit(const ItemDev* __restrict__ items, c
   r.slice = step - it.start_step;
   r.active = r.slice >= 0 && r.slice < jb.S;
   const int row = rg * RW + lr;
-  r.dst = it.wave + WaveLay<N>::at(row, t);
-  r.src = r.slice == 0 ? it.probe + (size_t)row * N + t : r.dst;
-  r.sstep = r.slice == 0 ? Cfg::T : WaveLay<N>::row_step(Cfg::T);
+  constexpr unsigned kBytes = (unsigned)N * N * sizeof(float2);
+  r.dst = buf_rsrc(it.wave, kBytes);
+  r.dst_off = Lay::at(row, t) * (int)sizeof(float2);
+  const bool probe = r.slice == 0;
+  r.src = buf_rsrc(probe ? it.probe : it.wave, kBytes);
+  r.src_off = probe ? (row * N + t) * (int)sizeof(float2) : r.dst_off;
+  r.src_step = (probe ? Cfg::T : Lay::row_step(Cfg::T)) * (int)sizeof(float2);
   r.np = jb.np;
   int pr = it.r0 + row;
   if (pr >= r.np) pr -= r.np;  // only reachable with wrap (validated on the host)
-  r.trow = jb.t + (size_t)(r.active ? r.slice : 0) * r.np * r.np + (size_t)pr * r.np;
+  const size_t np2 = (size_t)r.np * r.np;
+  r.trs = buf_rsrc(jb.t + (size_t)(r.active ? r.slice : 0) * np2, (unsigned)(np2 * sizeof(float2)));
+  r.t_off = pr * r.np * (int)sizeof(float2);
   r.c0 = it.c0;
   return r;
 }
 
-template <int N>
-__global__ void __launch_bounds__(RowCfg<N>::THREADS, TOTEM_ROWS_MINWAVES)
+template <int N, int V>
+__global__ void __launch_bounds__((RowCfg<N, V>::THREADS), TOTEM_ROWS_MINWAVES)
 k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step, const float2* __restrict__ tw) {
-  using Cfg = RowCfg<N>;
+  using Cfg = RowCfg<N, V>;
+  using PL = typename Cfg::PL;
   constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, K = TOTEM_ROWS_K;
-  constexpr int LLP = Cfg::LL + rows_lds_pad<N>();
+  constexpr int LLP = Cfg::LL + rows_lds_pad<N, V>();
   static_assert(Cfg::NBUF * RW * LLP * sizeof(float2) <= kLdsLimit, "rows-pass LDS above 64 KiB");
   __shared__ float2 lds[Cfg::NBUF * RW * LLP];
   float2* const lds1 = lds + (Cfg::NBUF - 1) * RW * LLP;
   int lr, t;
-  row_lane<N>(threadIdx.x, lr, t);
-  const RowLayT<rows_wave_local<N>()> lay{lr * LLP};
-  constexpr int DSTEP = WaveLay<N>::row_step(T);
+  row_lane<N, V>(threadIdx.x, lr, t);
+  const RowLayT<rows_wave_local<N, V>()> lay{lr * LLP};
+  constexpr int DSTEP = WaveLay<N, V>::row_step(T) * (int)sizeof(float2);
   float2 xa[E], xb[E];
-  RowUnit ua = row_unit<N, K>(items, jobs, step, 0, lr, t);
+  RowUnit ua = row_unit<N, V, K>(items, jobs, step, 0, lr, t);
   if (ua.active) {
 #pragma unroll
-    for (int v = 0; v < E; ++v) xa[v] = gld(ua.src + ua.sstep * v);
+    for (int v = 0; v < E; ++v) xa[v] = bld(ua.src, ua.src_off, ua.src_step * v);
   }
 #pragma unroll
   for (int u = 0; u < K; ++u) {
     RowUnit ub;
     if (u + 1 < K) {
-      ub = row_unit<N, K>(items, jobs, step, u + 1, lr, t);
+      ub = row_unit<N, V, K>(items, jobs, step, u + 1, lr, t);
       if (ub.active) {
 #pragma unroll
-        for (int v = 0; v < E; ++v) xb[v] = gld(ub.src + ub.sstep * v);
+        for (int v = 0; v < E; ++v) xb[v] = bld(ub.src, ub.src_off, ub.src_step * v);
       }
     }
     if (ua.active) {  // uniform over the workgroup
-      float2 tv[TOTEM_ROWS_TPREFETCH ? E : 1];
-      if constexpr (TOTEM_ROWS_TPREFETCH) {
+      float2 tv[Cfg::TPREFETCH ? E : 1];
+      if constexpr (Cfg::TPREFETCH) {
 #pragma unroll
         for (int v = 0; v < E; ++v) {
           int pc = ua.c0 + t + T * v;
           if (pc >= ua.np) pc -= ua.np;
-          tv[v] = gld(ua.trow + pc);
+          tv[v] = bld(ua.trs, ua.t_off + pc * (int)sizeof(float2), 0);
         }
       }
       if (ua.slice > 0) {
-        block_fft<N, +1, Cfg::PP, 0, Cfg::SPLIT>(xa, lds, lds1, lay, t, tw);
+        block_fft<N, +1, Cfg::PP, 0, Cfg::SPLIT, PL>(xa, lds, lds1, lay, t, tw);
         const float s = 1.0f / (float)(N * N);
 #pragma unroll
         for (int v = 0; v < E; ++v) xa[v] = cscale(xa[v], s);
       }
 #pragma unroll
       for (int v = 0; v < E; ++v) {
-        if constexpr (TOTEM_ROWS_TPREFETCH) {
+        if constexpr (Cfg::TPREFETCH) {
           xa[v] = cmul(xa[v], tv[v]);
         } else {
           int pc = ua.c0 + t + T * v;
           if (pc >= ua.np) pc -= ua.np;
-          xa[v] = cmul(xa[v], gld(ua.trow + pc));
+          xa[v] = cmul(xa[v], bld(ua.trs, ua.t_off + pc * (int)sizeof(float2), 0));
         }
       }
-      block_fft<N, -1, Cfg::PP, next_start<N, Cfg::PP>(0), Cfg::SPLIT>(xa, lds, lds1, lay, t, tw);
+      block_fft<N, -1, Cfg::PP, next_start<N, Cfg::PP, PL>(0), Cfg::SPLIT, PL>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
-      for (int v = 0; v < E; ++v) gst(ua.dst + DSTEP * v, xa[v]);
+      for (int v = 0; v < E; ++v) bst(ua.dst, ua.dst_off, DSTEP * v, xa[v]);
       if (u + 1 < K) __syncthreads();  // LDS reuse by the next unit
     }
     if (u + 1 < K) {
@@ -367,10 +426,10 @@ struct ColUnit {
   double* pbase;
 };
 
-template <int N, int K>
+template <int N, int V, int K>
 __device__ __forceinline__ ColUnit col_unit(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs,
                                             int step, int u) {
-  using Cfg = ColCfg<N>;
+  using Cfg = ColCfg<N, V>;
   constexpr int G = N / Cfg::C;
   ColUnit r;
   int item;
@@ -391,11 +450,13 @@ __device__ __forceinline__ ColUnit col_unit(const ItemDev* __restrict__ items, c
 // per workgroup).  A variant compiled without the detector code was measured
 // slower at every size (its register allocation gives fewer waves per SIMD:
 // at 1024 points 160 vs 128 VGPRs, 101 vs 89 us per launch of 26 probes).
-template <int N>
-__global__ void __launch_bounds__(ColCfg<N>::THREADS, TOTEM_COLS_MINWAVES)
+template <int N, int V>
+__global__ void __launch_bounds__((ColCfg<N, V>::THREADS), TOTEM_COLS_MINWAVES)
 k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step,
             const float2* __restrict__ tw) {
-  using Cfg = ColCfg<N>;
+  using Cfg = ColCfg<N, V>;
+  using PL = typename Cfg::PL;
+  using Lay = WaveLay<N, V>;
   constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C, K = TOTEM_COLS_K;
   constexpr int NW = (Cfg::THREADS + 63) / 64;
   constexpr int LC = Cfg::LC;
@@ -410,10 +471,10 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   constexpr unsigned kBytes = (unsigned)N * N * sizeof(float2);
   constexpr int kStep = T * N * (int)sizeof(float2);
   float2 xa[E], xb[E];
-  ColUnit ua = col_unit<N, K>(items, jobs, step, 0);
+  ColUnit ua = col_unit<N, V, K>(items, jobs, step, 0);
   // byte offsets of (row t, column): in the wave (WaveLay) and in row-major arrays (H, exit wave);
   // both advance by T*N elements per v
-  int voffa = WaveLay<N>::at(t, ua.cg * C + c) * (int)sizeof(float2);
+  int voffa = Lay::at(t, ua.cg * C + c) * (int)sizeof(float2);
   int hoffa = (t * N + ua.cg * C + c) * (int)sizeof(float2);
   if (ua.active) {
 #pragma unroll
@@ -424,8 +485,8 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
     ColUnit ub;
     int voffb = 0, hoffb = 0;
     if (u + 1 < K) {
-      ub = col_unit<N, K>(items, jobs, step, u + 1);
-      voffb = WaveLay<N>::at(t, ub.cg * C + c) * (int)sizeof(float2);
+      ub = col_unit<N, V, K>(items, jobs, step, u + 1);
+      voffb = Lay::at(t, ub.cg * C + c) * (int)sizeof(float2);
       hoffb = (t * N + ub.cg * C + c) * (int)sizeof(float2);
       if (ub.active) {
 #pragma unroll
@@ -441,7 +502,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
 #pragma unroll
         for (int v = 0; v < E; ++v) hv[v] = bld(hr, hoffa, v * kStep);
       }
-      block_fft<N, -1, Cfg::PP, 0, Cfg::SPLIT>(xa, lds, lds1, lay, t, tw);
+      block_fft<N, -1, Cfg::PP, 0, Cfg::SPLIT, PL>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
       for (int v = 0; v < E; ++v) xa[v] = cmul(xa[v], TOTEM_COLS_HPREFETCH ? hv[v] : bld(hr, hoffa, v * kStep));
       bool keep = true;
@@ -464,7 +525,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
         }
       }
       if (keep) {
-        block_fft<N, +1, Cfg::PP, next_start<N, Cfg::PP>(0), Cfg::SPLIT>(xa, lds, lds1, lay, t, tw);
+        block_fft<N, +1, Cfg::PP, next_start<N, Cfg::PP, PL>(0), Cfg::SPLIT, PL>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
         for (int v = 0; v < E; ++v) bst(ua.wr, voffa, v * kStep, xa[v]);
       }
@@ -568,12 +629,13 @@ k_fft_cols(const float2* __restrict__ src, float2* __restrict__ dst, int epi, fl
 
 // ex[j][c] = exp(-2 pi i kx1[c] x_j), ey[j][r] = exp(-2 pi i ky1[r] y_j); the product
 // k.x is formed and reduced modulo 1 in FP64 before the (FP64) sincospi.
+// Only the first nc entries of each table are formed (the accumulation reads no more).
 __global__ void k_phase(const double* __restrict__ tx, const double* __restrict__ ty, int n_terms,
-                        const double* __restrict__ kx1, const double* __restrict__ ky1, int np,
+                        const double* __restrict__ kx1, const double* __restrict__ ky1, int np, int nc,
                         float2* __restrict__ ex, float2* __restrict__ ey) {
   const int c = blockIdx.y * blockDim.x + threadIdx.x;
   const int j = blockIdx.x;
-  if (c >= np || j >= n_terms) return;
+  if (c >= nc || j >= n_terms) return;
   double fx = kx1[c] * tx[j];
   fx -= rint(fx);
   double fy = ky1[c] * ty[j];
@@ -585,10 +647,13 @@ __global__ void k_phase(const double* __restrict__ tx, const double* __restrict_
   ey[(size_t)j * np + c] = make_float2((float)cy, (float)-sy);
 }
 
+// Terms with a zero coefficient are skipped: the one-term classes of erf-weighted
+// slicing keep a single Gaussian (the branch is uniform: one class per wave).
 __device__ __forceinline__ double peng5(const double* p, double s2) {
   double f = 0.0;
 #pragma unroll
-  for (int q = 0; q < 5; ++q) f += p[2 * q] * exp(-p[2 * q + 1] * s2);
+  for (int q = 0; q < 5; ++q)
+    if (p[2 * q] != 0.0) f += p[2 * q] * exp(-p[2 * q + 1] * s2);
   return f;
 }
 __device__ __forceinline__ double lobato5(const double* p, double g2) {
@@ -622,44 +687,72 @@ __device__ __forceinline__ double2 form_factor(const PotClass& k, double s2, con
 
 // One slice: V_k(r,c) = PARAMETER * AperTrue * sum_classes f_cls(k) * sum_{j in cls} w_j ex_j[c] ey_j[r],
 // written at the ifftshifted position (the reference's quadrant swap) as complex64.
-// Per class the sum over terms is a complex matrix product, A[r][j] = ey_j[r]
-// times B[j][c] = w_j ex_j[c], accumulated on the matrix cores in FP32; the tile
-// is then weighted by the class's FP64 form factor per pixel.  One wave per
-// 16 x 16 pixel tile (every supported grid side is a multiple of 16).
-#ifndef TOTEM_ACC_PAIRS
-#define TOTEM_ACC_PAIRS 4
-#endif
-constexpr int kMfmaPairs = TOTEM_ACC_PAIRS;
-
-// v_mfma_f32_16x16x4f32 (4 terms per product, four real products per complex
-// one: Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br): lane l supplies A[l % 16][l / 16]
-// and B[l / 16][l % 16]; accumulator element i of lane l is C[4 (l / 16) + i][l % 16].
-// All loads of kMfmaPairs groups of four terms are issued before their products.
-// Measured per 80-slice 1024^2 build of 2000 terms per slice: 36.7 ms, against
-// 44.1 ms with 32 x 32 tiles (v_mfma_f32_32x32x2f32, 186 VGPRs, 2 waves/SIMD)
-// and 54.4 ms with VALU outer products over LDS-staged terms.
+//
+// Per class the sum over terms is a complex matrix product, A[r][j] = ey_j[r] times
+// B[j][c] = w_j ex_j[c], on the matrix cores in FP32 (v_mfma_f32_16x16x4f32, four
+// terms per product): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16], and
+// accumulator element i of lane l is C[4 (l / 16) + i][l % 16].  The four real
+// products are kept apart,
+//   P1 = Ar Br,  P2 = Ai Bi,  P3 = Ar Bi,  P4 = Ai Br  (summed over terms),
+// because the weights are real and the grid's frequencies come in pairs
+// (ky[N - r] = -ky[r], kx[N - c] = -kx[c]): then ey_j[N - r] = conj(ey_j[r]) and
+// ex_j[N - c] = conj(ex_j[c]), and one tile's products give four pixels,
+//   S(r, c)         = (P1 - P2) + i (P3 + P4)
+//   S(r, N - c)     = (P1 + P2) + i (P4 - P3)
+//   S(N - r, c)     = (P1 + P2) + i (P3 - P4)
+//   S(N - r, N - c) = (P1 - P2) - i (P3 + P4).
+// With `sym` set the tiles cover rows and columns [0, N/2 + 16) (the Nyquist row
+// and column, which have no pair, among them) and rows / columns r in
+// [1, N/2 - 16] also write their pair N - r: every pixel is written once.
+// Without it (a grid whose frequencies are not paired) the tiles cover the grid.
+// W waves share a tile, each taking every W-th group of 16 terms of a class; the
+// partial products are summed in LDS and each thread then weights one tile
+// position's pixels by the class's FP64 form factor at each of them.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-__global__ void __launch_bounds__(64)
+template <int W>
+__global__ void __launch_bounds__(64 * W)
 k_accumulate_mfma16(const float2* __restrict__ ex, const float2* __restrict__ ey, const double* __restrict__ wt,
                     const PotSeg* __restrict__ segs, int n_segs, const PotClass* __restrict__ classes,
                     const double* __restrict__ s2, const double* __restrict__ aper_true,
-                    const double* __restrict__ corr, double parameter, int np, float2* __restrict__ vshift) {
-  const int lane = threadIdx.x;
-  const int tiles = np / 16;
-  const int tr = blockIdx.x / tiles, tc = blockIdx.x - tr * tiles;
+                    const double* __restrict__ corr, double parameter, int np, int sym,
+                    float2* __restrict__ vshift) {
+  constexpr int NT = 64 * W, NO = 1024 / NT;  // threads; output pixels per thread
+  static_assert(NO * NT == 1024, "four pixels per tile position");
+  __shared__ float red[W * 4 * 4 * 64];      // [wave][product][element][lane]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int dt = sym ? np / 32 + 1 : np / 16;
+  const int tr = blockIdx.x / dt, tc = blockIdx.x - tr * dt;
   const int r0 = tr * 16, c0 = tc * 16;
   const int li = lane & 15, lk = lane >> 4;
   const size_t np2 = (size_t)np * np;
-  double vr[4], vi[4];
+  // this thread's tile position, its LDS source (lane, element) and output pixels
+  const int pos = threadIdx.x & 255;
+  const int pr = pos >> 4, pcl = pos & 15;
+  const int src = (pr >> 2) * 16 + pcl + 64 * (pr & 3);  // source lane + 64 * accumulator element
+  const int r = r0 + pr, c = c0 + pcl;
+  const int pair_hi = np / 2 - 16;
+  bool valid[NO];
+  bool cj_r[NO], cj_c[NO];
+  size_t pix[NO];
+  double s2v[NO], vr[NO], vi[NO];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) vr[i] = vi[i] = 0.0;
+  for (int o = 0; o < NO; ++o) {
+    const int quad = (threadIdx.x >> 8) + (NT >> 8) * o;  // bit 0: column pair, bit 1: row pair
+    cj_c[o] = quad & 1;
+    cj_r[o] = (quad >> 1) & 1;
+    valid[o] = (!cj_r[o] || (sym && r >= 1 && r <= pair_hi)) && (!cj_c[o] || (sym && c >= 1 && c <= pair_hi));
+    const int R = cj_r[o] ? np - r : r, C = cj_c[o] ? np - c : c;
+    pix[o] = (size_t)R * np + C;
+    s2v[o] = valid[o] ? s2[pix[o]] : 0.0;
+    vr[o] = vi[o] = 0.0;
+  }
   for (int sgi = 0; sgi < n_segs; ++sgi) {
     const PotSeg sg = segs[sgi];
-    floatx4 cr = {}, ci = {};
-    for (int j = sg.begin; j < sg.end; j += 4 * kMfmaPairs) {
-      float2 a[kMfmaPairs], b[kMfmaPairs];
+    floatx4 p1 = {}, p2 = {}, p3 = {}, p4 = {};
+    for (int j = sg.begin + 16 * wave; j < sg.end; j += 16 * W) {
+      float2 a[4], b[4];
 #pragma unroll
-      for (int u = 0; u < kMfmaPairs; ++u) {
+      for (int u = 0; u < 4; ++u) {
         const int jj = j + 4 * u + lk;
         a[u] = b[u] = make_float2(0.f, 0.f);
         if (jj < sg.end) {
@@ -670,31 +763,51 @@ k_accumulate_mfma16(const float2* __restrict__ ex, const float2* __restrict__ ey
         }
       }
 #pragma unroll
-      for (int u = 0; u < kMfmaPairs; ++u) {
-        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].x, cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].y, ci, 0, 0, 0);
-        cr = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u].y, b[u].y, cr, 0, 0, 0);
-        ci = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, ci, 0, 0, 0);
+      for (int u = 0; u < 4; ++u) {
+        p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].x, p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].y, p2, 0, 0, 0);
+        p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].y, p3, 0, 0, 0);
+        p4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, p4, 0, 0, 0);
       }
     }
-    const PotClass& cls = classes[sg.cls];
+    float* const mine = red + wave * 1024;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int r = r0 + 4 * lk + i, c = c0 + li;
-      const size_t pix = (size_t)r * np + c;
-      const double2 f = form_factor(cls, s2[pix], corr, pix, np2);
-      vr[i] += f.x * cr[i] - f.y * ci[i];
-      vi[i] += f.x * ci[i] + f.y * cr[i];
+      mine[0 * 256 + 64 * i + lane] = p1[i];
+      mine[1 * 256 + 64 * i + lane] = p2[i];
+      mine[2 * 256 + 64 * i + lane] = p3[i];
+      mine[3 * 256 + 64 * i + lane] = p4[i];
+    }
+    __syncthreads();
+    float q1 = 0.f, q2 = 0.f, q3 = 0.f, q4 = 0.f;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+      const float* o = red + w * 1024 + src;
+      q1 += o[0];
+      q2 += o[256];
+      q3 += o[512];
+      q4 += o[768];
+    }
+    __syncthreads();  // red is rewritten by the next class
+    const PotClass& cls = classes[sg.cls];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+      if (!valid[o]) continue;
+      const double sre = cj_r[o] == cj_c[o] ? (double)(q1 - q2) : (double)(q1 + q2);
+      const double sim = (double)(cj_c[o] ? -q3 : q3) + (double)(cj_r[o] ? -q4 : q4);
+      const double2 f = form_factor(cls, s2v[o], corr, pix[o], np2);
+      vr[o] += f.x * sre - f.y * sim;
+      vi[o] += f.x * sim + f.y * sre;
     }
   }
   const int half = np / 2;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r = r0 + 4 * lk + i, c = c0 + li;
-    const size_t pix = (size_t)r * np + c;
-    const double scale = parameter * aper_true[pix];
-    const int rs = (r + half) % np, cs = (c + half) % np;
-    vshift[(size_t)rs * np + cs] = make_float2((float)(vr[i] * scale), (float)(vi[i] * scale));
+  for (int o = 0; o < NO; ++o) {
+    if (!valid[o]) continue;
+    const int R = (int)(pix[o] / np), C = (int)(pix[o] - (size_t)R * np);
+    const double scale = parameter * aper_true[pix[o]];
+    const int rs = (R + half) % np, cs = (C + half) % np;
+    vshift[(size_t)rs * np + cs] = make_float2((float)(vr[o] * scale), (float)(vi[o] * scale));
   }
 }
 
@@ -709,42 +822,36 @@ __global__ void k_fill_f64(double* __restrict__ p, long n, double v) {
 
 // --------------------------------------------------------------------- launchers
 
-template <int N> void launch_pass_rows(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
-                                       int step) {
-  using Cfg = RowCfg<N>;
-  constexpr int K = TOTEM_ROWS_K;
-  static_assert((N / Cfg::RW) % K == 0, "row groups per probe must be a multiple of K");
-  const dim3 grid((unsigned)(n_items * (N / Cfg::RW) / K));
-  hipLaunchKernelGGL(k_pass_rows<N>, grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, tw_table(N));
-}
-template <int N> void launch_pass_cols(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
-                                       int step) {
-  using Cfg = ColCfg<N>;
-  constexpr int K = TOTEM_COLS_K;
-  static_assert((N / Cfg::C) % K == 0, "column groups per probe must be a multiple of K");
-  const dim3 grid((unsigned)(n_items * (N / Cfg::C) / K));
-  hipLaunchKernelGGL((k_pass_cols<N>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, tw_table(N));
-}
-template <int N> void launch_fft2d(hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi,
-                                   float sigma) {
-  using R = RowCfg<N>;
-  using Cc = ColCfg<N>;
-  const long rows = (long)batch * N;
-  const dim3 rgrid((unsigned)((rows + R::RW - 1) / R::RW));
-  const float2* tw = tw_table(N);
-  if (sign < 0) hipLaunchKernelGGL((k_fft_rows<N, -1>), rgrid, dim3(R::THREADS), 0, st, data, rows, tw);
-  else hipLaunchKernelGGL((k_fft_rows<N, +1>), rgrid, dim3(R::THREADS), 0, st, data, rows, tw);
-  const dim3 cgrid((unsigned)(batch * (N / Cc::C)));
-  if (sign < 0) hipLaunchKernelGGL((k_fft_cols<N, -1>), cgrid, dim3(Cc::THREADS), 0, st, data, dst, epi, sigma, tw);
-  else hipLaunchKernelGGL((k_fft_cols<N, +1>), cgrid, dim3(Cc::THREADS), 0, st, data, dst, epi, sigma, tw);
-}
-template <int N> int col_groups() { return N / ColCfg<N>::C; }
+#define TOTEM_FOR_SIZES(X) X(64) X(128) X(256) X(320) X(512) X(768) X(1024) X(1536) X(2048)
 
-template <int N> void twiddle_host_n(std::vector<float2>& out) {
+// Twiddle tables of every plan in one static device array (no allocation on any
+// path that launches the passes); each table starts at an even entry so that its
+// float4 reads stay 16-byte aligned.
+constexpr int tw_even(int n) { return (n + 1) & ~1; }
+template <class Q, class... PLS> struct TwOffset {
+  static constexpr int value = -(1 << 28);  // not registered
+};
+template <class Q, class P0, class... PLS> struct TwOffset<Q, P0, PLS...> {
+  static constexpr int value =
+      std::is_same<Q, P0>::value ? 0 : tw_even(plan_twiddle_len<P0>()) + TwOffset<Q, PLS...>::value;
+};
+template <class... PLS> struct PlanSet {
+  static constexpr int total = (0 + ... + tw_even(plan_twiddle_len<PLS>()));
+  template <class Q> static constexpr int offset() {
+    constexpr int o = TwOffset<Q, PLS...>::value;
+    static_assert(o >= 0, "plan without a twiddle table");
+    return o;
+  }
+};
+using AllPlans = PlanSet<Plan<64>, Plan<128>, Plan<256>, Plan<320>, Plan<512>, Plan<768>, Plan<1024>, Plan<1536>,
+                         Plan<2048>, PlanB<768>>;
+__device__ __align__(16) float2 g_twiddles[AllPlans::total];
+
+template <class PL> void twiddle_host_plan(std::vector<float2>& out) {
   out.clear();
   int p = 1;
-  for (int i = 0; i < Plan<N>::NRAD; ++i) {
-    const int r = Plan<N>::radix(i);
+  for (int i = 0; i < PL::NRAD; ++i) {
+    const int r = PL::radix(i);
     if (p > 1)
       for (int k = 0; k < p; ++k)
         for (int q = 0; q < r; ++q) {
@@ -754,59 +861,96 @@ template <int N> void twiddle_host_n(std::vector<float2>& out) {
     p *= r;
   }
 }
-
-#define TOTEM_FOR_SIZES(X) X(64) X(128) X(256) X(320) X(512) X(768) X(1024) X(1536) X(2048)
-
-void twiddle_host(int n, std::vector<float2>& out) {
-  out.clear();
-#define X(S) if (n == S) { twiddle_host_n<S>(out); return; }
-  TOTEM_FOR_SIZES(X)
-#undef X
+template <class PL> static void put_table(std::vector<float2>& all) {
+  std::vector<float2> h;
+  twiddle_host_plan<PL>(h);
+  std::copy(h.begin(), h.end(), all.begin() + AllPlans::offset<PL>());
+}
+template <class... PLS> static void put_tables(std::vector<float2>& all, PlanSet<PLS...>*) {
+  (put_table<PLS>(all), ...);
 }
 
-// Every plan's twiddle table lives in one static device array (no allocation on
-// any path that launches the passes); each table starts at an even entry so that
-// its float4 reads stay 16-byte aligned.
-constexpr int tw_even(int n) { return (n + 1) & ~1; }
-constexpr int kTwTotal = 0
-#define X(S) + tw_even(twiddle_len<S>())
-    TOTEM_FOR_SIZES(X)
-#undef X
-    ;
-__device__ __align__(16) float2 g_twiddles[kTwTotal];
-
-static int twiddle_offset(int n) {
-  int off = 0;
-#define X(S) if (n == S) return off; off += tw_even(twiddle_len<S>());
-  TOTEM_FOR_SIZES(X)
-#undef X
-  return -1;
-}
-
-const float2* tw_table(int n) {
-  const int off = twiddle_offset(n);
-  require(off >= 0, "no FFT plan for n=" + std::to_string(n), TOTEM_ERR_UNSUPPORTED);
+// Device address of g_twiddles on the current device, filled on first use.
+static float2* twiddle_base() {
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
   static std::mutex mu;
-  static std::map<int, float2*> base;  // per device: address of g_twiddles, filled once
+  static std::map<int, float2*> base;
   std::lock_guard<std::mutex> lk(mu);
   auto it = base.find(dev);
   if (it == base.end()) {
     float2* p = nullptr;
     HIP_OK(hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(g_twiddles)));
-    std::vector<float2> all(kTwTotal, make_float2(0.f, 0.f)), h;
-    int o = 0;
-#define X(S)                                                   \
-  twiddle_host_n<S>(h);                                        \
-  std::copy(h.begin(), h.end(), all.begin() + o);              \
-  o += tw_even(twiddle_len<S>());
-    TOTEM_FOR_SIZES(X)
-#undef X
+    std::vector<float2> all(AllPlans::total, make_float2(0.f, 0.f));
+    put_tables(all, static_cast<AllPlans*>(nullptr));
     HIP_OK(hipMemcpy(p, all.data(), all.size() * sizeof(float2), hipMemcpyHostToDevice));
     it = base.emplace(dev, p).first;
   }
-  return it->second + off;
+  return it->second;
+}
+template <class PL> static const float2* tw_for() { return twiddle_base() + AllPlans::offset<PL>(); }
+
+void twiddle_host(int n, std::vector<float2>& out) {
+  out.clear();
+#define X(S) if (n == S) { twiddle_host_plan<Plan<S>>(out); return; }
+  TOTEM_FOR_SIZES(X)
+#undef X
+}
+
+const float2* tw_table(int n) {
+#define X(S) if (n == S) return tw_for<Plan<S>>();
+  TOTEM_FOR_SIZES(X)
+#undef X
+  require(false, "no FFT plan for n=" + std::to_string(n), TOTEM_ERR_UNSUPPORTED);
+  return nullptr;
+}
+
+template <int N, int V> void launch_pass_rows(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
+                                              int step) {
+  using Cfg = RowCfg<N, V>;
+  constexpr int K = TOTEM_ROWS_K;
+  static_assert((N / Cfg::RW) % K == 0, "row groups per probe must be a multiple of K");
+  const dim3 grid((unsigned)(n_items * (N / Cfg::RW) / K));
+  hipLaunchKernelGGL((k_pass_rows<N, V>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step,
+                     tw_for<typename Cfg::PL>());
+}
+template <int N, int V> void launch_pass_cols(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
+                                              int step) {
+  using Cfg = ColCfg<N, V>;
+  constexpr int K = TOTEM_COLS_K;
+  static_assert((N / Cfg::C) % K == 0, "column groups per probe must be a multiple of K");
+  const dim3 grid((unsigned)(n_items * (N / Cfg::C) / K));
+  hipLaunchKernelGGL((k_pass_cols<N, V>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step,
+                     tw_for<typename Cfg::PL>());
+}
+template <int N> void launch_fft2d(hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi,
+                                   float sigma) {
+  using R = RowCfg<N>;
+  using Cc = ColCfg<N>;
+  const long rows = (long)batch * N;
+  const dim3 rgrid((unsigned)((rows + R::RW - 1) / R::RW));
+  const float2* tw = tw_for<Plan<N>>();
+  if (sign < 0) hipLaunchKernelGGL((k_fft_rows<N, -1>), rgrid, dim3(R::THREADS), 0, st, data, rows, tw);
+  else hipLaunchKernelGGL((k_fft_rows<N, +1>), rgrid, dim3(R::THREADS), 0, st, data, rows, tw);
+  const dim3 cgrid((unsigned)(batch * (N / Cc::C)));
+  if (sign < 0) hipLaunchKernelGGL((k_fft_cols<N, -1>), cgrid, dim3(Cc::THREADS), 0, st, data, dst, epi, sigma, tw);
+  else hipLaunchKernelGGL((k_fft_cols<N, +1>), cgrid, dim3(Cc::THREADS), 0, st, data, dst, epi, sigma, tw);
+}
+
+template <class PL> static std::string plan_radices() {
+  std::string r;
+  for (int i = 0; i < PL::NRAD; ++i) r += (i ? "." : "") + std::to_string(PL::radix(i));
+  return r;
+}
+template <int N, int V> static std::string variant_desc() {
+  using R = RowCfg<N, V>;
+  using C = ColCfg<N, V>;
+  return "rows " + plan_radices<typename R::PL>() + " on " + std::to_string(R::T) + " lanes x " +
+         std::to_string(R::RW) + " rows" + (R::SPLIT ? " (half-buffer exchanges)" : "") +
+         (R::TPREFETCH ? "" : " (transmission loaded after the inverse transform)") + "; columns " +
+         plan_radices<typename C::PL>() + " on " + std::to_string(C::T) + " lanes x " + std::to_string(C::C) +
+         " columns" + (C::SPLIT ? " (half-buffer exchanges)" : "") + "; tiles " +
+         std::to_string(WaveLay<N, V>::TR) + " x " + std::to_string(WaveLay<N, V>::TC);
 }
 
 bool size_supported(int n) {
@@ -816,21 +960,61 @@ bool size_supported(int n) {
   return false;
 }
 
-int detector_groups(int n) {
-#define X(S) if (n == S) return col_groups<S>();
+int pass_variant_count(int n) {
+#define X(S) if (n == S) return pass_variants<S>();
   TOTEM_FOR_SIZES(X)
 #undef X
   return 0;
 }
 
-void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step) {
-#define X(S) if (n == S) { launch_pass_rows<S>(st, items, n_items, jobs, step); return; }
+std::string pass_variant_desc(int n, int v) {
+#define X(S)                                                          \
+  if (n == S) {                                                       \
+    if constexpr (pass_variants<S>() > 1)                             \
+      if (v == 1) return variant_desc<S, 1>();                        \
+    return variant_desc<S, 0>();                                      \
+  }
+  TOTEM_FOR_SIZES(X)
+#undef X
+  return "";
+}
+
+int detector_groups(int n, int v) {
+#define X(S)                                                          \
+  if (n == S) {                                                       \
+    if constexpr (pass_variants<S>() > 1)                             \
+      if (v == 1) return S / ColCfg<S, 1>::C;                         \
+    return S / ColCfg<S, 0>::C;                                       \
+  }
+  TOTEM_FOR_SIZES(X)
+#undef X
+  return 0;
+}
+
+int detector_groups_max(int n) {
+  int g = 0;
+  for (int v = 0; v < pass_variant_count(n); ++v) g = std::max(g, detector_groups(n, v));
+  return g;
+}
+
+void pass_rows(int n, int v, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step) {
+#define X(S)                                                                    \
+  if (n == S) {                                                                 \
+    if constexpr (pass_variants<S>() > 1)                                       \
+      if (v == 1) return launch_pass_rows<S, 1>(st, items, n_items, jobs, step); \
+    return launch_pass_rows<S, 0>(st, items, n_items, jobs, step);              \
+  }
   TOTEM_FOR_SIZES(X)
 #undef X
 }
 
-void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step) {
-#define X(S) if (n == S) { launch_pass_cols<S>(st, items, n_items, jobs, step); return; }
+void pass_cols(int n, int v, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step) {
+#define X(S)                                                                    \
+  if (n == S) {                                                                 \
+    if constexpr (pass_variants<S>() > 1)                                       \
+      if (v == 1) return launch_pass_cols<S, 1>(st, items, n_items, jobs, step); \
+    return launch_pass_cols<S, 0>(st, items, n_items, jobs, step);              \
+  }
   TOTEM_FOR_SIZES(X)
 #undef X
 }
@@ -856,22 +1040,37 @@ void fft2d(int n, hipStream_t st, float2* data, float2* dst, int batch, int sign
 #undef X
 }
 
+bool hermitian_pairs(const double* kx1, const double* ky1, int np) {
+  if (np % 32 != 0) return false;
+  for (int r = 1; r <= np / 2 - 16; ++r)
+    if (ky1[np - r] != -ky1[r] || kx1[np - r] != -kx1[r]) return false;
+  return true;
+}
+
+int phase_columns(int np, bool sym) { return sym ? np / 2 + 16 : np; }
+
 void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_terms, const double* kx1,
-                  const double* ky1, int np, float2* ex, float2* ey) {
+                  const double* ky1, int np, bool sym, float2* ex, float2* ey) {
   if (n_terms <= 0) return;
-  hipLaunchKernelGGL(k_phase, dim3(n_terms, (np + 255) / 256), dim3(256), 0, st, tx, ty, n_terms, kx1, ky1, np,
+  const int nc = phase_columns(np, sym);
+  hipLaunchKernelGGL(k_phase, dim3(n_terms, (nc + 255) / 256), dim3(256), 0, st, tx, ty, n_terms, kx1, ky1, np, nc,
                      ex, ey);
 }
 
+#ifndef TOTEM_ACC_WAVES
+#define TOTEM_ACC_WAVES 4  // waves sharing one potential tile
+#endif
+
 void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,
                 const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
-                double parameter, int np, float2* vshift) {
+                double parameter, int np, bool sym, float2* vshift) {
 #define X(S) static_assert(S % 16 == 0, "potential tiles need grid sides that are multiples of 16");
   TOTEM_FOR_SIZES(X)
 #undef X
-  const int tiles = np / 16;
-  hipLaunchKernelGGL(k_accumulate_mfma16, dim3((unsigned)(tiles * tiles)), dim3(64), 0, st, ex, ey, wt, segs, n_segs,
-                     classes, s2, aper_true, corr, parameter, np, vshift);
+  const int dt = sym ? np / 32 + 1 : np / 16;
+  constexpr int W = TOTEM_ACC_WAVES;
+  hipLaunchKernelGGL((k_accumulate_mfma16<W>), dim3((unsigned)(dt * dt)), dim3(64 * W), 0, st, ex, ey, wt, segs,
+                     n_segs, classes, s2, aper_true, corr, parameter, np, sym ? 1 : 0, vshift);
 }
 
 }  // namespace totem

@@ -4,6 +4,7 @@
 #include "kernels.h"
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace totem {
@@ -16,14 +17,18 @@ const float2* tw_table(int n);
 void twiddle_host(int n, std::vector<float2>& out);
 
 bool size_supported(int n);   // FFT lengths with a compiled radix plan
-int detector_groups(int n);   // column groups per probe in k_pass_cols
+// Pass variants ("plans") of n-point grids: schedules of the rows and columns passes
+// (radix plans, rows / columns per workgroup, exchange form).  Both passes of a run
+// must use the same variant (it fixes the in-flight wave layout).
+int pass_variant_count(int n);
+std::string pass_variant_desc(int n, int variant);
+int detector_groups(int n, int variant);  // column groups per probe in k_pass_cols
+int detector_groups_max(int n);           // over the variants (sizes detector partials)
 
 // One slice step of every listed item (slice = step - item.start_step; items outside
-// [0, S) of their job are skipped).
-void pass_rows(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
-// det: some item of the launch may be at a detector slice (mid-slice or exit face);
-// false selects the variant without detector code.
-void pass_cols(int n, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
+// [0, S) of their job are skipped), with pass variant `variant` of n.
+void pass_rows(int n, int variant, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
+void pass_cols(int n, int variant, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
 // Detector sums of the items items[list[e]] (list may be null: items[e]), e < n, into their
 // job's det; lmax1 and amax bound (L + 1) and A over the jobs involved.
 void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs, int groups,
@@ -34,11 +39,17 @@ void fft2d(int n, hipStream_t st, float2* data, float2* dst, int batch, int sign
 // p[0..n) = value (small fill kernels; graph-capture safe, unlike memset nodes)
 void fill(hipStream_t st, float2* p, long n, float re, float im);
 void fill(hipStream_t st, double* p, long n, double v);
+// True when the grid's frequencies pair up (ky1[np - r] = -ky1[r], kx1[np - c] = -kx1[c]
+// for 1 <= r, c <= np/2 - 16) and np is a multiple of 32: the accumulation then
+// forms a quarter of the tiles and writes each one's mirrored pixels too.
+bool hermitian_pairs(const double* kx1, const double* ky1, int np);
+// Entries per phase table the accumulation reads (np/2 + 16 with paired frequencies).
+int phase_columns(int np, bool sym);
 void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_terms, const double* kx1,
-                  const double* ky1, int np, float2* ex, float2* ey);
+                  const double* ky1, int np, bool sym, float2* ex, float2* ey);
 void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,
                 const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
-                double parameter, int np, float2* vshift);
+                double parameter, int np, bool sym, float2* vshift);
 
 // Specimen slicing on the device (specimen.hip).
 void displace(hipStream_t st, int n, const double* xyz, const double* rms, uint64_t seed, int config, double lo,
@@ -56,9 +67,12 @@ struct TermKeyArgs {
   int* counts = nullptr;
 };
 void term_keys(hipStream_t st, const TermKeyArgs& a);
-size_t sort_pairs_temp_bytes(int n, int bits);  // stable radix sort of (key, value) pairs on bits [0, bits)
-hipError_t sort_pairs(hipStream_t st, void* temp, size_t temp_bytes, const unsigned* keys_in, unsigned* keys_out,
-                      const unsigned* vals_in, unsigned* vals_out, int n, int bits);
+// Stable LSD radix sort of n (key, value) pairs on key bits [0, bits), 8 bits per
+// pass, ping-ponging between (keys, vals) and (keys2, vals2); returns the values
+// buffer that holds the sorted order.  temp: sort_pairs_temp_bytes(n) bytes.
+size_t sort_pairs_temp_bytes(int n);
+unsigned* sort_pairs(hipStream_t st, void* temp, unsigned* keys, unsigned* keys2, unsigned* vals, unsigned* vals2,
+                     int n, int bits);
 void term_gather(hipStream_t st, int n, int K, const unsigned* vals, const int* term_atom, const double* wslot,
                  const double* xyz, double* tx, double* ty, double* wt);
 

@@ -12,9 +12,10 @@
 // Wave slots (wave [N][N] complex64 + FP64 detector partials) live in one arena
 // split into pages of the largest slot size; a page serves one grid side at a
 // time and returns to the shared pool when its last slot is released.  Admission
-// is first come, first served over all grid sides: when the oldest waiting job
-// cannot get a single slot, the other sides stop taking new work until pages
-// drain to it, so no side starves another.
+// is first come, first served over all grid sides, with one job holding at most
+// half the pages while others wait; when a waiting job with nothing in flight
+// cannot get a single slot, the jobs behind it stop taking new work until pages
+// drain to it, so no job starves.
 //
 // Results do not depend on what else is in flight: every probe's arithmetic and
 // its detector reduction are the same as in a run of its job alone.
@@ -59,6 +60,7 @@ struct Job {
 
 struct Group {
   int n = 0;
+  int plan = 0;                       // pass variant of this side, fixed when the side is created
   hipStream_t st = nullptr;
   int step = 0;
   size_t slot_bytes = 0, wave_bytes = 0;
@@ -102,7 +104,7 @@ struct totem_server {
   size_t slot_bytes(int n, size_t* wave) const {
     const size_t w = totem::align_up((size_t)n * n * sizeof(float2));
     if (wave) *wave = w;
-    return w + totem::align_up((size_t)(max_mid + 1) * max_aper * totem::detector_groups(n) * sizeof(double));
+    return w + totem::align_up((size_t)(max_mid + 1) * max_aper * totem::detector_groups_max(n) * sizeof(double));
   }
   int meta_ints() const { return totem::srv::kMaxSlices + std::max(max_mid, 1); }
 };
@@ -117,6 +119,7 @@ Group& group_of(Server& s, int n) {
   if (it != s.groups.end()) return it->second;
   Group& g = s.groups[n];
   g.n = n;
+  g.plan = current_plan(n);
   g.slot_bytes = s.slot_bytes(n, &g.wave_bytes);
   g.per_page = (int)(s.page_bytes / g.slot_bytes);
   HIP_OK(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
@@ -148,16 +151,32 @@ void release_slot(Server& s, Group& g, int slot) {
   s.free_pages.push_back(page);
 }
 
-// Admits waiting jobs' probes, oldest job first.  Returns when the oldest job
-// gets no slot at all (other sides then wait for pages to drain) or nothing waits.
+// Admits waiting jobs' probes, oldest job first.  While other jobs wait, a job holds
+// at most half the arena's pages (in slots of its side), so one large job cannot
+// occupy the whole arena for its full run; a job that reached that share lets the
+// jobs behind it in.  A job with no probe in flight that gets no slot stops the
+// pass: the jobs behind it wait until pages drain to it, so no job starves.
 void admit(Server& s) {
-  while (!s.order.empty()) {
-    Job& J = s.jobs.at(s.order.front());
+  for (auto it = s.order.begin(); it != s.order.end();) {
+    Job& J = s.jobs.at(*it);
     Group& g = group_of(s, J.j.n);
-    const long want = J.total - J.next;
+    const long held = J.next - J.done;  // probes of J in flight
+    long want = J.total - J.next;
+    if (s.order.size() > 1) {
+      const long share = (long)std::max(1, s.n_pages / 2) * g.per_page;
+      want = std::min(want, share - held);
+    }
+    if (want <= 0) {  // at its share: the jobs behind it go next
+      ++it;
+      continue;
+    }
     while ((long)g.free_slots.size() < want && !s.free_pages.empty()) take_page(s, g);
     const int k = (int)std::min<long>(want, (long)g.free_slots.size());
-    if (k == 0) return;  // head of line blocked: its side frees slots as it runs, or pages drain to it
+    if (k == 0) {
+      if (held == 0) return;  // starved: pages drain to it before anyone behind it is admitted
+      ++it;
+      continue;
+    }
     Cohort c;
     c.job = J.id;
     c.start = g.step;
@@ -174,8 +193,8 @@ void admit(Server& s) {
     J.next += k;
     g.cohorts.push_back(std::move(c));
     g.dirty = true;
-    if (J.next < J.total) return;  // partially admitted: wait for slots
-    s.order.pop_front();
+    if (J.next == J.total) it = s.order.erase(it);
+    else ++it;
   }
 }
 
@@ -238,13 +257,14 @@ void step_group(Server& s, Group& g) {
   const int cnt = (int)g.host_items.size();
   const ItemDev* items = g.items_dev.as<ItemDev>();
   const JobDev* jobs = s.jobs_dev.as<JobDev>();
-  pass_rows(g.n, g.st, items, cnt, jobs, g.step);
-  pass_cols(g.n, g.st, items, cnt, jobs, g.step);
+  pass_rows(g.n, g.plan, g.st, items, cnt, jobs, g.step);
+  pass_cols(g.n, g.plan, g.st, items, cnt, jobs, g.step);
   bool removed = false;
   for (Cohort& c : g.cohorts) {
     Job& J = s.jobs.at(c.job);
     if (g.step - c.start != J.j.slices - 1) continue;
-    reduce_det(g.st, items + c.offset, nullptr, c.count, jobs, detector_groups(g.n), s.max_mid + 1, s.max_aper);
+    reduce_det(g.st, items + c.offset, nullptr, c.count, jobs, detector_groups(g.n, g.plan), s.max_mid + 1,
+               s.max_aper);
     for (int slot : c.slots) release_slot(s, g, slot);
     J.done += c.count;
     if (J.done == J.total) {

@@ -7,12 +7,14 @@
 // run per phonon configuration on the device, on terms that carry their atom
 // and form-factor class: displaced positions -> (slice, class) keys -> stable
 // radix sort -> term arrays grouped by slice and class, the layout the
-// potential kernels read.  Random numbers: Philox4x32-10 (Salmon et al., SC'11),
+// potential kernels read.  The sort is a stable LSD radix sort written here (8-bit
+// digits: per-tile digit counts, an exclusive scan, an in-order scatter).  Random
+// numbers: Philox4x32-10 (Salmon et al., SC'11),
 // counter = (atom, configuration, attempt, lane), key = seed; normals by
 // Box-Muller in double precision.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "launch.h"
@@ -193,6 +195,150 @@ __global__ void k_fill_i32(int n, int v, int* __restrict__ p) {
   if (i < n) p[i] = v;
 }
 
+// ------------------------------------------------------------ stable radix sort
+//
+// Tiles of kSortTile = 1024 elements, one wave each; element e * 64 + lane of a
+// tile is lane `lane`'s e-th, so a wave walks its tile in index order.  Per pass
+// (digit d = bits [shift, shift + 8) of the key):
+//   hist[d * G + tile] = elements of the tile with digit d
+//   exclusive scan of hist in that (digit-major, tile-minor) order
+//   scatter: an element goes to hist[d * G + tile] + (elements of its tile before
+//            it with the same digit)
+// which keeps equal digits in index order: the sort is stable, and LSD passes over
+// the digits give the order of the whole key with ties in index order.
+constexpr int kSortTile = 1024, kRadix = 256, kScanChunk = 1024;
+
+__global__ void __launch_bounds__(64) k_radix_hist(const unsigned* __restrict__ keys, int n, int shift, int G,
+                                                   unsigned* __restrict__ hist) {
+  __shared__ unsigned bins[kRadix];
+  const int lane = threadIdx.x, tile = blockIdx.x;
+#pragma unroll
+  for (int q = 0; q < kRadix / 64; ++q) bins[lane + 64 * q] = 0u;
+  __syncthreads();
+  for (int e = 0; e < kSortTile / 64; ++e) {
+    const long i = (long)tile * kSortTile + e * 64 + lane;
+    if (i < n) atomicAdd(&bins[(keys[i] >> shift) & (kRadix - 1)], 1u);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kRadix / 64; ++q) hist[(size_t)(lane + 64 * q) * G + tile] = bins[lane + 64 * q];
+}
+
+// Exclusive scan of 256 values, one per thread of a 256-thread block; `total` gets
+// their sum.  lds: 4 words.
+__device__ __forceinline__ unsigned block_exclusive_scan(unsigned v, unsigned* lds, unsigned& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) lds[wave] = x;
+  __syncthreads();
+  unsigned off = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const unsigned t = lds[w];
+    if (w < wave) off += t;
+    total += t;
+  }
+  __syncthreads();
+  return off + x - v;
+}
+
+// sums[b] = sum of data[b * kScanChunk, (b + 1) * kScanChunk)
+__global__ void __launch_bounds__(256) k_scan_sums(const unsigned* __restrict__ data, int m,
+                                                   unsigned* __restrict__ sums) {
+  __shared__ unsigned lds[4];
+  const long base = (long)blockIdx.x * kScanChunk + 4 * threadIdx.x;
+  unsigned v = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k < m) v += data[base + k];
+  unsigned total;
+  (void)block_exclusive_scan(v, lds, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// sums[0, nb) -> exclusive prefix sums, in place (one block, 1024 entries per round)
+__global__ void __launch_bounds__(256) k_scan_top(unsigned* __restrict__ sums, int nb) {
+  __shared__ unsigned lds[4];
+  unsigned carry = 0;
+  for (int base = 0; base < nb; base += 1024) {
+    const int i0 = base + 4 * threadIdx.x;
+    unsigned v[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = i0 + k < nb ? sums[i0 + k] : 0u;
+      s += v[k];
+    }
+    unsigned total;
+    unsigned run = carry + block_exclusive_scan(s, lds, total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (i0 + k < nb) sums[i0 + k] = run;
+      run += v[k];
+    }
+    carry += total;
+  }
+}
+
+// data -> exclusive prefix sums, each chunk offset by its scanned sums[b]
+__global__ void __launch_bounds__(256) k_scan_apply(unsigned* __restrict__ data, int m,
+                                                    const unsigned* __restrict__ sums) {
+  __shared__ unsigned lds[4];
+  const long base = (long)blockIdx.x * kScanChunk + 4 * threadIdx.x;
+  unsigned v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = base + k < m ? data[base + k] : 0u;
+    s += v[k];
+  }
+  unsigned total;
+  unsigned run = sums[blockIdx.x] + block_exclusive_scan(s, lds, total);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < m) data[base + k] = run;
+    run += v[k];
+  }
+}
+
+__global__ void __launch_bounds__(64) k_radix_scatter(const unsigned* __restrict__ keys_in,
+                                                      const unsigned* __restrict__ vals_in,
+                                                      unsigned* __restrict__ keys_out, unsigned* __restrict__ vals_out,
+                                                      int n, int shift, int G, const unsigned* __restrict__ offs) {
+  __shared__ unsigned bins[kRadix];
+  const int lane = threadIdx.x, tile = blockIdx.x;
+#pragma unroll
+  for (int q = 0; q < kRadix / 64; ++q) bins[lane + 64 * q] = offs[(size_t)(lane + 64 * q) * G + tile];
+  __syncthreads();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int e = 0; e < kSortTile / 64; ++e) {
+    const long i = (long)tile * kSortTile + e * 64 + lane;
+    const bool ok = i < n;
+    const unsigned key = ok ? keys_in[i] : 0u;
+    const int d = (int)((key >> shift) & (kRadix - 1));
+    unsigned long long todo = __ballot(ok);
+    unsigned dst = 0;
+    while (todo) {  // one digit value of this row of 64 elements per round
+      const int leader = __ffsll((long long)todo) - 1;
+      const int dl = __shfl(d, leader, 64);
+      const unsigned long long same = __ballot(ok && d == dl);
+      const unsigned base = bins[dl];
+      if (ok && d == dl) dst = base + (unsigned)__popcll(same & below);
+      if (lane == leader) bins[dl] = base + (unsigned)__popcll(same);
+      todo &= ~same;
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (ok) {
+      keys_out[dst] = key;
+      vals_out[dst] = vals_in[i];
+    }
+  }
+}
+
 __global__ void k_term_gather(int n, int K, const unsigned* __restrict__ vals, const int* __restrict__ term_atom,
                               const double* __restrict__ wslot, const double* __restrict__ xyz,
                               double* __restrict__ tx, double* __restrict__ ty, double* __restrict__ wt) {
@@ -223,17 +369,29 @@ void term_keys(hipStream_t st, const TermKeyArgs& a) {
                        a.discon, a.keys, a.vals, a.wslot, a.counts);
 }
 
-size_t sort_pairs_temp_bytes(int n, int bits) {
-  size_t bytes = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                           (const unsigned*)nullptr, (unsigned*)nullptr, n, 0, bits);
-  return bytes;
+size_t sort_pairs_temp_bytes(int n) {
+  const size_t G = ((size_t)std::max(n, 1) + kSortTile - 1) / kSortTile;
+  const size_t m = kRadix * G, nb = (m + kScanChunk - 1) / kScanChunk;
+  return (m + nb + 64) * sizeof(unsigned);
 }
 
-hipError_t sort_pairs(hipStream_t st, void* temp, size_t temp_bytes, const unsigned* keys_in, unsigned* keys_out,
-                      const unsigned* vals_in, unsigned* vals_out, int n, int bits) {
-  size_t bytes = temp_bytes;
-  return hipcub::DeviceRadixSort::SortPairs(temp, bytes, keys_in, keys_out, vals_in, vals_out, n, 0, bits, st);
+unsigned* sort_pairs(hipStream_t st, void* temp, unsigned* keys, unsigned* keys2, unsigned* vals, unsigned* vals2,
+                     int n, int bits) {
+  const int G = (n + kSortTile - 1) / kSortTile;
+  const int m = kRadix * G, nb = (m + kScanChunk - 1) / kScanChunk;
+  unsigned* const hist = static_cast<unsigned*>(temp);
+  unsigned* const sums = hist + m;
+  unsigned *ks = keys, *kd = keys2, *vs = vals, *vd = vals2;
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(G), dim3(64), 0, st, ks, n, shift, G, hist);
+    hipLaunchKernelGGL(k_scan_sums, dim3(nb), dim3(256), 0, st, hist, m, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, sums, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(256), 0, st, hist, m, sums);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(G), dim3(64), 0, st, ks, vs, kd, vd, n, shift, G, hist);
+    std::swap(ks, kd);
+    std::swap(vs, vd);
+  }
+  return vs;
 }
 
 void term_gather(hipStream_t st, int n, int K, const unsigned* vals, const int* term_atom, const double* wslot,

@@ -36,7 +36,8 @@ EXPORTS = (
     "totem_last_error", "totem_abi_version", "totem_fft_size_supported", "totem_build_transmission",
     "totem_potential_plan_stats",
     "totem_stem_workspace_bytes", "totem_stem_run", "totem_stem_run_static", "totem_kernel_times", "totem_fft2d",
-    "totem_stem_preferred_batch", "totem_stem_autotune", "totem_hrtem_images", "totem_tcc_image", "totem_weighted_pattern",
+    "totem_stem_preferred_batch", "totem_stem_autotune", "totem_stem_plan_count", "totem_stem_plan",
+    "totem_stem_set_plan", "totem_hrtem_images", "totem_tcc_image", "totem_weighted_pattern",
     "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
     "totem_server_destroy",
     "totem_specimen_create", "totem_specimen_transmission", "totem_specimen_positions", "totem_specimen_destroy",
@@ -148,6 +149,12 @@ def lib():
         L.totem_stem_autotune.restype = _I
         L.totem_stem_preferred_batch.argtypes = [_I]
         L.totem_stem_preferred_batch.restype = _I
+        L.totem_stem_plan_count.argtypes = [_I]
+        L.totem_stem_plan_count.restype = _I
+        L.totem_stem_plan.argtypes = [_I, ctypes.POINTER(_I), ctypes.c_char_p, ctypes.c_size_t]
+        L.totem_stem_plan.restype = _I
+        L.totem_stem_set_plan.argtypes = [_I, _I]
+        L.totem_stem_set_plan.restype = _I
         L.totem_specimen_create.argtypes = [ctypes.POINTER(SpecimenDesc), ctypes.POINTER(_VP)]
         L.totem_specimen_transmission.argtypes = [_VP, ctypes.c_ulonglong, _I, _VP, _VP]
         L.totem_specimen_positions.argtypes = [_VP, ctypes.c_ulonglong, _I, _DP]
@@ -273,15 +280,17 @@ def build_transmission(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx,
 def potential_plan_stats(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx, ky, s_2, aper_true, sigma,
                          parameter, **kw):
     """Host-side plan of build_transmission's arguments (no device work): a dict with the
-    form-factor ``classes``, the ``terms`` over all slices and ``split`` (Peng rows split into
-    one class per Gaussian term)."""
+    form-factor ``classes``, the ``terms`` over all slices, ``split`` (Peng rows split into
+    one class per Gaussian term) and ``paired`` (kx, ky pair up, so a quarter of the
+    reciprocal-space tiles is accumulated and mirrored)."""
     d, keep = _potential_desc(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx, ky, s_2, aper_true, sigma,
                               parameter, **kw)
-    classes, split = ctypes.c_int(0), ctypes.c_int(0)
+    classes, split, paired = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
     terms = ctypes.c_longlong(0)
     check(lib().totem_potential_plan_stats(ctypes.byref(d), ctypes.byref(classes), ctypes.byref(terms),
-                                           ctypes.byref(split)))
-    return {"classes": classes.value, "terms": terms.value, "split": bool(split.value)}
+                                           ctypes.byref(split), ctypes.byref(paired)))
+    return {"classes": classes.value, "terms": terms.value, "split": bool(split.value),
+            "paired": bool(paired.value)}
 
 
 def _potential_desc(paraflag, np_side, slices, atom_slice, proj_coff_mat, kx, ky, s_2, aper_true, sigma, parameter,
@@ -398,11 +407,30 @@ def weighted_pattern(fields_ptr, n, weights, shift=True):
 
 
 def autotune(n, slices=8):
-    """Measure and adopt the probes-per-launch for n x n grids; returns (batch, probe.slice/s)."""
+    """Measure every pass plan of n x n grids with probes-per-launch around the cache-fit rule and
+    adopt the fastest pair on the current device; returns (batch, probe.slice/s) (see :func:`plan`)."""
     b = _I()
     r = ctypes.c_double()
     check(lib().totem_stem_autotune(int(n), int(slices), ctypes.byref(b), ctypes.byref(r)))
     return b.value, r.value
+
+
+def plan_count(n):
+    """Pass plans (schedules of the rows and columns passes) compiled for n x n grids."""
+    return int(lib().totem_stem_plan_count(int(n)))
+
+
+def plan(n):
+    """(index, description) of the pass plan in use for n x n grids on the current device."""
+    p = _I()
+    buf = ctypes.create_string_buffer(512)
+    check(lib().totem_stem_plan(int(n), ctypes.byref(p), buf, ctypes.c_size_t(len(buf))))
+    return p.value, buf.value.decode()
+
+
+def set_plan(n, index):
+    """Use pass plan ``index`` for n x n grids on the current device (runs started afterwards)."""
+    check(lib().totem_stem_set_plan(int(n), int(index)))
 
 
 def stem_workspace_bytes(job, batch):
