@@ -230,6 +230,49 @@ def test_stem_myprojectfun_matches_oracle(engine, flag, M, mid):
     assert np.all(px == 0) and np.all(py == 0)
 
 
+@pytest.mark.parametrize("flag,M,mid", [("l", 1, (2,)), ("n", 3, (2, 5))])
+def test_drop_in_library_symbol_matches_oracle(engine, flag, M, mid):
+    """``myProjectfun`` exported by libwaveCudalobato.so / libwaveCudaPengAndCorr.so (the symbol the
+    mex gateway binds, waveCuda.h:10-21) called through ctypes: the same images as the engine entry
+    point bit for bit, within 1e-4 per pixel of the FP64 oracle."""
+    print("case drop-in symbol %s flag=%s M=%d mid=%s" % (stem.mex_name(flag), flag, M, mid), flush=True)
+    cfg, inp = _stem_case(flag, M=M, mid=mid)
+    res_d, mid_d, px, py = stem.run(inp, drop_in=True)
+    res_e, mid_e, _, _ = stem.run(inp)
+    np.testing.assert_array_equal(res_d, res_e)
+    np.testing.assert_array_equal(mid_d, mid_e)
+    res_o, mid_o = _oracle_stem(inp)
+    _check_images(res_d, res_o, "exit-face detectors")
+    _check_images(mid_d, mid_o, "mid-slice detectors")
+    assert np.all(px == 0) and np.all(py == 0)
+
+
+@pytest.mark.parametrize("flag", ["l", "n"])
+def test_drop_in_potential_library_symbol_matches_oracle(engine, flag):
+    """``myProjectfun`` of libwaveCudalobatoP.so / libwaveCudaPengAndCorrP.so: the transmission
+    function within 1e-6 relative L2 of the oracle, equal to the engine entry point's."""
+    print("case drop-in symbol %s" % stem.mex_name(flag, True), flush=True)
+    cfg, inp, _ = _potential_case(flag)
+    cfg.width_red = cfg.hight_red = 1
+    _, _, px, py = stem.run(inp, potential_only=True, drop_in=True)
+    _, _, pxe, pye = stem.run(inp, potential_only=True)
+    np.testing.assert_array_equal(px, pxe)
+    np.testing.assert_array_equal(py, pye)
+    _, to = _oracle_potential(inp)
+    S, npx = cfg.n_slices, cfg.np_side
+    err = _rel_l2((px + 1j * py).T.reshape(S, npx, npx), to)
+    print("  t rel L2 %.3e" % err)
+    assert err <= 1e-6
+
+
+def test_drop_in_library_symbol_reports_errors(engine):
+    """A window that leaves the potential: the drop-in symbol returns nothing (as the reference's
+    does) and leaves the message in totem_last_error(), which the wrapper raises."""
+    cfg, inp = _stem_case("l", scan=(3, 2), step=60, begin=(20, 30))
+    with pytest.raises(engine.TotemError, match="window leaves the potential"):
+        stem.run(inp, drop_in=True)
+
+
 @pytest.mark.parametrize("flag,n,np_side,S", [("n", 320, 512, 4), ("n", 768, 1024, 4), ("l", 1536, 2048, 2)])
 def test_stem_mixed_radix_grids_match_oracle(engine, flag, n, np_side, S):
     """Windows of 320 (radix 5), 768 and 1536 (radix 3) points -- the plans whose rows pass exchanges

@@ -402,9 +402,9 @@ void potential_run(const PotDev& p, float2* t_dev, double* v_re, double* v_im, h
   for (int s = 0; s < S; ++s) {
     const int nt = (int)(p.term_off[s + 1] - p.term_off[s]);
     float2* ts = t_dev + (size_t)s * np2;
-    phase_tables(st, p.tx + p.term_off[s], p.ty + p.term_off[s], nt, p.kx1, p.ky1, np, p.sym, dex.as<float2>(),
-                 dey.as<float2>());
-    accumulate(st, dex.as<float2>(), dey.as<float2>(), p.wt + p.term_off[s], p.segs + p.seg_off[s],
+    phase_tables(st, p.tx + p.term_off[s], p.ty + p.term_off[s], p.wt + p.term_off[s], nt, p.kx1, p.ky1, np, p.sym,
+                 dex.as<float2>(), dey.as<float2>());
+    accumulate(st, dex.as<float2>(), dey.as<float2>(), p.segs + p.seg_off[s],
                (int)(p.seg_off[s + 1] - p.seg_off[s]), p.classes, p.s2, p.aper, p.corr, p.parameter, np, p.sym, ts);
     if (want_v) {
       fft2d(np, st, ts, ts, 1, +1, 0, 0.f);

@@ -627,12 +627,14 @@ k_fft_cols(const float2* __restrict__ src, float2* __restrict__ dst, int epi, fl
 
 // ------------------------------------------------------------------- potential build
 
-// ex[j][c] = exp(-2 pi i kx1[c] x_j), ey[j][r] = exp(-2 pi i ky1[r] y_j); the product
-// k.x is formed and reduced modulo 1 in FP64 before the (FP64) sincospi.
-// Only the first nc entries of each table are formed (the accumulation reads no more).
-__global__ void k_phase(const double* __restrict__ tx, const double* __restrict__ ty, int n_terms,
-                        const double* __restrict__ kx1, const double* __restrict__ ky1, int np, int nc,
-                        float2* __restrict__ ex, float2* __restrict__ ey) {
+// ex[j][c] = w_j exp(-2 pi i kx1[c] x_j), ey[j][r] = exp(-2 pi i ky1[r] y_j); the
+// product k.x is formed and reduced modulo 1 in FP64 before the (FP64) sincospi, and
+// the term's weight is applied in FP64 before the rounding to FP32.  Only the first
+// nc entries of each table are formed (the accumulation reads no more).
+__global__ void k_phase(const double* __restrict__ tx, const double* __restrict__ ty,
+                        const double* __restrict__ wt, int n_terms, const double* __restrict__ kx1,
+                        const double* __restrict__ ky1, int np, int nc, float2* __restrict__ ex,
+                        float2* __restrict__ ey) {
   const int c = blockIdx.y * blockDim.x + threadIdx.x;
   const int j = blockIdx.x;
   if (c >= nc || j >= n_terms) return;
@@ -643,7 +645,8 @@ __global__ void k_phase(const double* __restrict__ tx, const double* __restrict_
   double sx, cx, sy, cy;
   sincospi(2.0 * fx, &sx, &cx);
   sincospi(2.0 * fy, &sy, &cy);
-  ex[(size_t)j * np + c] = make_float2((float)cx, (float)-sx);
+  const double w = wt[j];
+  ex[(size_t)j * np + c] = make_float2((float)(w * cx), (float)(-w * sx));
   ey[(size_t)j * np + c] = make_float2((float)cy, (float)-sy);
 }
 
@@ -689,7 +692,7 @@ __device__ __forceinline__ double2 form_factor(const PotClass& k, double s2, con
 // written at the ifftshifted position (the reference's quadrant swap) as complex64.
 //
 // Per class the sum over terms is a complex matrix product, A[r][j] = ey_j[r] times
-// B[j][c] = w_j ex_j[c], on the matrix cores in FP32 (v_mfma_f32_16x16x4f32, four
+// B[j][c] = ex_j[c] (the weight w_j already in ex), on the matrix cores in FP32 (v_mfma_f32_16x16x4f32, four
 // terms per product): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16], and
 // accumulator element i of lane l is C[4 (l / 16) + i][l % 16].  The four real
 // products are kept apart,
@@ -708,11 +711,13 @@ __device__ __forceinline__ double2 form_factor(const PotClass& k, double s2, con
 // W waves share a tile, each taking every W-th group of 16 terms of a class; the
 // partial products are summed in LDS and each thread then weights one tile
 // position's pixels by the class's FP64 form factor at each of them.
+#ifndef TOTEM_ACC_MINWAVES
+#define TOTEM_ACC_MINWAVES 1  // minimum waves per SIMD the accumulation is compiled for
+#endif
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 template <int W>
-__global__ void __launch_bounds__(64 * W)
-k_accumulate_mfma16(const float2* __restrict__ ex, const float2* __restrict__ ey, const double* __restrict__ wt,
-                    const PotSeg* __restrict__ segs, int n_segs, const PotClass* __restrict__ classes,
+__global__ void __launch_bounds__(64 * W, TOTEM_ACC_MINWAVES)
+k_accumulate_mfma16(const float2* __restrict__ ex, const float2* __restrict__ ey, const PotSeg* __restrict__ segs, int n_segs, const PotClass* __restrict__ classes,
                     const double* __restrict__ s2, const double* __restrict__ aper_true,
                     const double* __restrict__ corr, double parameter, int np, int sym,
                     float2* __restrict__ vshift) {
@@ -757,9 +762,7 @@ k_accumulate_mfma16(const float2* __restrict__ ex, const float2* __restrict__ ey
         a[u] = b[u] = make_float2(0.f, 0.f);
         if (jj < sg.end) {
           a[u] = ey[(size_t)jj * np + r0 + li];
-          const float w = (float)wt[jj];
-          const float2 e = ex[(size_t)jj * np + c0 + li];
-          b[u] = make_float2(w * e.x, w * e.y);
+          b[u] = ex[(size_t)jj * np + c0 + li];
         }
       }
 #pragma unroll
@@ -1049,19 +1052,19 @@ bool hermitian_pairs(const double* kx1, const double* ky1, int np) {
 
 int phase_columns(int np, bool sym) { return sym ? np / 2 + 16 : np; }
 
-void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_terms, const double* kx1,
-                  const double* ky1, int np, bool sym, float2* ex, float2* ey) {
+void phase_tables(hipStream_t st, const double* tx, const double* ty, const double* wt, int n_terms,
+                  const double* kx1, const double* ky1, int np, bool sym, float2* ex, float2* ey) {
   if (n_terms <= 0) return;
   const int nc = phase_columns(np, sym);
-  hipLaunchKernelGGL(k_phase, dim3(n_terms, (nc + 255) / 256), dim3(256), 0, st, tx, ty, n_terms, kx1, ky1, np, nc,
-                     ex, ey);
+  hipLaunchKernelGGL(k_phase, dim3(n_terms, (nc + 255) / 256), dim3(256), 0, st, tx, ty, wt, n_terms, kx1, ky1, np,
+                     nc, ex, ey);
 }
 
 #ifndef TOTEM_ACC_WAVES
-#define TOTEM_ACC_WAVES 4  // waves sharing one potential tile
+#define TOTEM_ACC_WAVES 8  // waves sharing one potential tile
 #endif
 
-void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,
+void accumulate(hipStream_t st, const float2* ex, const float2* ey, const PotSeg* segs, int n_segs,
                 const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
                 double parameter, int np, bool sym, float2* vshift) {
 #define X(S) static_assert(S % 16 == 0, "potential tiles need grid sides that are multiples of 16");
@@ -1069,8 +1072,8 @@ void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double
 #undef X
   const int dt = sym ? np / 32 + 1 : np / 16;
   constexpr int W = TOTEM_ACC_WAVES;
-  hipLaunchKernelGGL((k_accumulate_mfma16<W>), dim3((unsigned)(dt * dt)), dim3(64 * W), 0, st, ex, ey, wt, segs,
-                     n_segs, classes, s2, aper_true, corr, parameter, np, sym ? 1 : 0, vshift);
+  hipLaunchKernelGGL((k_accumulate_mfma16<W>), dim3((unsigned)(dt * dt)), dim3(64 * W), 0, st, ex, ey, segs, n_segs,
+                     classes, s2, aper_true, corr, parameter, np, sym ? 1 : 0, vshift);
 }
 
 }  // namespace totem

@@ -45,9 +45,9 @@ void fill(hipStream_t st, double* p, long n, double v);
 bool hermitian_pairs(const double* kx1, const double* ky1, int np);
 // Entries per phase table the accumulation reads (np/2 + 16 with paired frequencies).
 int phase_columns(int np, bool sym);
-void phase_tables(hipStream_t st, const double* tx, const double* ty, int n_terms, const double* kx1,
-                  const double* ky1, int np, bool sym, float2* ex, float2* ey);
-void accumulate(hipStream_t st, const float2* ex, const float2* ey, const double* wt, const PotSeg* segs, int n_segs,
+void phase_tables(hipStream_t st, const double* tx, const double* ty, const double* wt, int n_terms,
+                  const double* kx1, const double* ky1, int np, bool sym, float2* ex, float2* ey);
+void accumulate(hipStream_t st, const float2* ex, const float2* ey, const PotSeg* segs, int n_segs,
                 const PotClass* classes, const double* s2, const double* aper_true, const double* corr,
                 double parameter, int np, bool sym, float2* vshift);
 

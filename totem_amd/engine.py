@@ -195,13 +195,32 @@ def _f64(a):
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
 
 
+_drop_in = {}
+
+
+def drop_in_library(name):
+    """ctypes handle of the drop-in library ``lib<name>.so`` (``name`` a key of ENTRY), whose one
+    export is ``myProjectfun`` with the reference's prototype (``waveCuda.h:10-21``)."""
+    h = _drop_in.get(name)
+    if h is None:
+        lib()  # the engine first: the drop-in library then binds to this same libtotem.so
+        h = ctypes.CDLL(os.path.join(os.path.dirname(CORE), "lib%s.so" % name))
+        h.myProjectfun.argtypes = _PROJECT_ARGTYPES
+        h.myProjectfun.restype = None
+        _drop_in[name] = h
+    return h
+
+
 def my_project_fun(entry, my_real_w, my_imag_w, aper, atom_slice, absorp_n, atom_slice_i, absorp_n_i,
                    proj_coff_mat, proj_coff_mat_i, kx, ky, s_2, my_real_const, my_img_const, gfsf,
                    Height, Width, Depth, P_Height, P_Width, Slices, step, beginRow, beginCol, width_red,
                    sigma, w_num, aper_num, parameter, AperTrue, mid_ceng, ceng_len, aper2,
-                   series_n_corr, series_n_i_corr, ele_n_corr, ele_n_i_corr, corr_info_matrix, cim_len):
+                   series_n_corr, series_n_i_corr, ele_n_corr, ele_n_i_corr, corr_info_matrix, cim_len,
+                   drop_in=None):
     """Call one engine entry point with the 39 inputs of myProjectfun (``waveCuda.h:10-21``);
-    beginRow/beginCol are 0-based here, as in the C prototype.  Returns
+    beginRow/beginCol are 0-based here, as in the C prototype.  With ``drop_in`` (a key of ENTRY)
+    the call goes through that drop-in library's ``myProjectfun`` symbol instead, which returns
+    nothing and leaves a failure in totem_last_error().  Returns
     (return_result [aper_num, Depth], mid_ceng_mat [ceng_len, aper_num, Depth],
     potentialx, potentialy [Slices, P_Height, P_Width])."""
     arrays = [_f64(v) for v in (my_real_w, my_imag_w, aper, atom_slice, absorp_n, atom_slice_i, absorp_n_i,
@@ -215,13 +234,18 @@ def my_project_fun(entry, my_real_w, my_imag_w, aper, atom_slice, absorp_n, atom
     np2 = P_Height * P_Width
     px = np.zeros((Slices, P_Height, P_Width)) if potential_only else np.zeros(1)
     py = np.zeros((Slices, P_Height, P_Width)) if potential_only else np.zeros(1)
-    fn = getattr(lib(), entry)
+    fn = drop_in_library(drop_in).myProjectfun if drop_in else getattr(lib(), entry)
     rc = fn(*[_dp(a) for a in arrays], int(Height), int(Width), int(Depth), int(P_Height), int(P_Width),
             int(Slices), int(step), int(beginRow), int(beginCol), int(width_red), float(sigma), int(w_num),
             int(aper_num), float(parameter), _dp(ap_true), mid.ctypes.data_as(_IP), int(ceng_len),
             *[_dp(a) for a in tail], int(cim_len), _dp(res), _dp(midm) if midm.size else _dp(np.zeros(1)),
             _dp(px), _dp(py))
-    check(rc)
+    if drop_in:
+        msg = lib().totem_last_error().decode()
+        if msg:
+            raise TotemError(-1, msg)
+    else:
+        check(rc)
     del np2
     return res, midm, (px if potential_only else None), (py if potential_only else None)
 
@@ -230,12 +254,14 @@ def wave_cuda(name, my_real_w, my_img_w, aper, atom_slice, absorp_n2, atom_slice
               proj_coff_mat_i, Kx, Ky, S_2, my_real_const, my_img_const, gfsf, Height, Width, Depth, P_Height,
               P_Width, Slices, Step, beginRow, beginCol, width_red, sigma, w_num, aper_num, parameter, aperTrue,
               absorp_flag, absorp_i_flag, mid_ceng, ceng_len, aper2, seriesn_corr, seriesn_i_corr, elen_corr,
-              elen_i_corr, corrinfo_matrix, cimlen):
+              elen_i_corr, corrinfo_matrix, cimlen, drop_in=False):
     """The mex function ``waveCuda*_<v>`` with its 41 arguments in ``cuda_stem.m:97`` order
     (``waveCuda.cpp:18-79``: absorption flags select NULL, cimlen==0 drops the correction table,
     1-based beginRow/beginCol).  Returns the four mex outputs shaped as MATLAB sees them:
     myresul [Depth, aper_num], mid_ceng_mat [Depth*aper_num, ceng_len],
-    potentialx/potentialy [P_Height*P_Width, Slices] (zeros for the STEM variants)."""
+    potentialx/potentialy [P_Height*P_Width, Slices] (zeros for the STEM variants).
+    With ``drop_in`` the call goes through ``myProjectfun`` of ``lib<name>.so``, as the mex
+    gateway does."""
     if name not in ENTRY:
         raise ValueError("unknown mex name %r" % name)
     res, midm, px, py = my_project_fun(
@@ -243,7 +269,8 @@ def wave_cuda(name, my_real_w, my_img_w, aper, atom_slice, absorp_n2, atom_slice
         atom_slice_i, absorp_n_i2 if absorp_i_flag == 1 else None, proj_coff_mat, proj_coff_mat_i, Kx, Ky, S_2,
         my_real_const, my_img_const, gfsf, Height, Width, Depth, P_Height, P_Width, Slices, Step, beginRow - 1,
         beginCol - 1, width_red, sigma, w_num, aper_num, parameter, aperTrue, mid_ceng, ceng_len, aper2,
-        seriesn_corr, seriesn_i_corr, elen_corr, elen_i_corr, corrinfo_matrix if cimlen != 0 else None, cimlen)
+        seriesn_corr, seriesn_i_corr, elen_corr, elen_i_corr, corrinfo_matrix if cimlen != 0 else None, cimlen,
+        drop_in=name if drop_in else None)
     L = max(int(ceng_len), 0)
     myresul = res.T.copy()                                   # [Depth, aper_num]
     mid_mat = midm.reshape(L, aper_num * Depth).T.copy()     # [Depth*aper_num, ceng_len]

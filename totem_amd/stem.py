@@ -215,10 +215,11 @@ def mex_name(paraflag, potential_only=False):
     return base + ("P" if potential_only else "")
 
 
-def run(inp, potential_only=False):
-    """Run the engine through the mex-shaped entry point; returns the four mex outputs."""
+def run(inp, potential_only=False, drop_in=False):
+    """Run the engine through the mex-shaped entry point; returns the four mex outputs.  With
+    ``drop_in`` the call goes through ``myProjectfun`` of the matching drop-in library."""
     from . import engine
-    return engine.wave_cuda(mex_name(inp.cfg.paraflag, potential_only), *mex_args(inp))
+    return engine.wave_cuda(mex_name(inp.cfg.paraflag, potential_only), *mex_args(inp), drop_in=drop_in)
 
 
 # ------------------------------------------------------------------ workloads
