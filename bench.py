@@ -202,6 +202,64 @@ def run_serving(n_requests, rate, seed, arena_gib, budget_gb):
             "arena_GiB": arena_gib, "hbm_budget_GB": budget_gb, "seed": seed}
 
 
+def run_erf_build(reps=3):
+    """Device build of the frozen-phonon specimen's transmission (80 000 atoms, Peng+corr, 80 slices
+    of 1024^2) with erf-weighted slicing at multiceng 0, 0.5 and 1: displacement, slicing, sort and
+    potential of one phonon configuration, best of ``reps`` after one untimed build."""
+    import torch
+    from dataclasses import replace
+    from totem_amd import engine, stem
+    cfg0 = stem.perovskite_c2(scan=4)
+    out = {}
+    for m in (0.0, 0.5, 1.0):
+        cfg = replace(cfg0, multiceng=m)
+        sp = stem.device_specimen(cfg, sigma_by_z=stem.C2_PHONON_SIGMA)
+        t = torch.empty((cfg.n_slices, cfg.np_side, cfg.np_side), dtype=torch.complex64, device="cuda")
+        st = torch.cuda.current_stream().cuda_stream
+        sp.transmission(1, 0, t.data_ptr(), stream=st)
+        torch.cuda.synchronize()
+        best = None
+        for k in range(reps):
+            t0 = time.perf_counter()
+            sp.transmission(1, k + 1, t.data_ptr(), stream=st)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        out["%g" % m] = best * 1e3
+        sp.close()
+        del t
+        torch.cuda.empty_cache()
+    return {"workload": "SrTiO3 20x20x40 cells (80 000 atoms), Peng+corr, 80 slices of 1024^2, one phonon "
+                        "configuration: displacement + slicing + sort + potential + t",
+            "ms_per_config_by_multiceng": out}
+
+
+def run_drop_in():
+    """One ``myProjectfun`` call through the drop-in library libwaveCudaPengAndCorr.so with the mex
+    argument list of the frozen-phonon specimen (80 000 atoms, static snapshot, Peng+corr): host
+    rows in, 1024^2 potential built, 64x64 probes of 512^2 windows through 80 slices, detector images
+    back on the host.  This is the PCIe-inclusive form (the reference's contract); the second call
+    is reported, the first includes the library's one-time set-up."""
+    from dataclasses import replace
+    from totem_amd import stem
+    cfg = replace(stem.perovskite_c2(scan=64), n=512)
+    t0 = time.perf_counter()
+    inp = stem.prepare(cfg)
+    prep = time.perf_counter() - t0
+    walls, outs = [], []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        outs.append(stem.run(inp, drop_in=True)[0])
+        walls.append(time.perf_counter() - t0)
+    M, D, S = len(inp.gfsf), cfg.depth, cfg.n_slices
+    return {"library": "libwaveCudaPengAndCorr.so", "symbol": "myProjectfun",
+            "workload": "SrTiO3 20x20x40 cells static snapshot, Peng+corr, 1024^2 potential, 80 slices, "
+                        "64x64 probes of 512^2 windows",
+            "s_per_call": walls[1], "first_call_s": walls[0], "host_prepare_s": prep,
+            "probe_slices_per_s": M * D * S / walls[1], "repeat_bit_identical": bool(np.array_equal(*outs)),
+            "includes": "host rows -> device, potential build, propagation, allocation, images -> host"}
+
+
 def run_uneven(n_jobs, seed, arena_gib):
     """Uneven batch: n_jobs seeded perovskite jobs of mixed grid sides (256^2-1024^2, 768^2 with a
     radix-3 stage), thickness, defocus and scan size, all in one served run, against the same jobs
@@ -243,7 +301,7 @@ def main():
     ap.add_argument("--steps", type=int, default=8, help="timed phonon configurations")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=["c2", "c1"], default="c2")
-    ap.add_argument("--graph", type=int, default=0, help="capture the slice loop in a hipGraph")
+    ap.add_argument("--graph", type=int, default=1, help="capture the slice loop in a hipGraph (0: launch directly)")
     ap.add_argument("--scan", type=int, default=64, help="scan side of the 1024^2 workload (64: 64x64 probes)")
     ap.add_argument("--batch", type=int, default=0,
                     help="probes in flight per launch (0: measured by the engine's autotuner)")
@@ -266,7 +324,10 @@ def main():
     ap.add_argument("--hrtem-images", type=int, default=256,
                     help="images per call of the batch HRTEM leg (0: skip)")
     ap.add_argument("--hrtem-only", action="store_true", help="run only the batch HRTEM leg (profiling)")
-    ap.add_argument("--only", choices=["stress", "uneven", "serving", "hrtem"],
+    ap.add_argument("--erf-build", type=int, default=1, help="time the erf-weighted specimen build (0: skip)")
+    ap.add_argument("--drop-in", type=int, default=1,
+                    help="time one myProjectfun call through the drop-in library (0: skip)")
+    ap.add_argument("--only", choices=["stress", "uneven", "serving", "hrtem", "erf", "dropin"],
                     help="run only this leg and print it alone (profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-traffic kernel for "
@@ -297,6 +358,12 @@ def main():
         return
     if args.only == "stress":
         print(json.dumps({"precision_stress": run_precision_stress(max(args.stress_scan, 1))}), flush=True)
+        return
+    if args.only == "erf":
+        print(json.dumps({"erf_weighted_build": run_erf_build()}), flush=True)
+        return
+    if args.only == "dropin":
+        print(json.dumps({"drop_in_call": run_drop_in()}), flush=True)
         return
     if args.only == "uneven":
         print(json.dumps({"uneven_batch": run_uneven(max(args.uneven_jobs, 1), seed=1004, arena_gib=0.25)}),
@@ -491,7 +558,8 @@ def main():
             "propagation_probe_slices_per_s": len(positions) * M * S / prop_cpu}
         out["oracle_max_rel_err"] = rel
         out["oracle_error_covers"] = "GPU potential build + propagation vs the FP64 C++ oracle from the same atoms"
-    if args.uneven_jobs > 0 or args.serve_requests > 0 or args.hrtem_images > 0 or args.stress_scan > 0:
+    if (args.uneven_jobs > 0 or args.serve_requests > 0 or args.hrtem_images > 0 or args.stress_scan > 0 or
+            args.erf_build or args.drop_in):
         if not args.host_slicing:
             spec.close()
         del ws, t, probes, det
@@ -500,6 +568,10 @@ def main():
         out["hrtem_batch"] = run_hrtem_batch(args.hrtem_images, 8, seed=1006)
     if args.stress_scan > 0:
         out["precision_stress"] = run_precision_stress(args.stress_scan)
+    if args.erf_build:
+        out["erf_weighted_build"] = run_erf_build()
+    if args.drop_in:
+        out["drop_in_call"] = run_drop_in()
     if args.uneven_jobs > 0:
         out["uneven_batch"] = run_uneven(args.uneven_jobs, seed=1004, arena_gib=0.25)
     if args.serve_requests > 0:

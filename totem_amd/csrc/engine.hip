@@ -1289,7 +1289,8 @@ static void project_impl(const ProjectArgs& a, int paraflag, bool potential_only
   long batch = (long)(fr / 2 / std::max<size_t>(per, 1));
   batch = std::max<long>(1, std::min<long>(batch, std::min<long>((long)M * D, preferred_batch(n))));
   DevBuf ws(stem_ws_bytes(n, S, A, L, (int)batch, (long)M * D));
-  run_stem_impl(j, (int)batch, 0, ws.p, ws.bytes, st);
+  // more than one batch: the slice loop is captured once and replayed per batch
+  run_stem_impl(j, (int)batch, (long)M * D > batch ? TOTEM_RUN_GRAPH : 0, ws.p, ws.bytes, st);
   std::vector<double> det(det_n);
   HIP_OK(hipMemcpyAsync(det.data(), ddet.p, det_n * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));

@@ -35,6 +35,12 @@
 namespace totem {
 namespace srv {
 
+#ifndef TOTEM_SERVER_AHEAD
+#define TOTEM_SERVER_AHEAD 4
+#endif
+#ifndef TOTEM_SERVER_SHARE
+#define TOTEM_SERVER_SHARE 2  // a job holds at most 1/TOTEM_SERVER_SHARE of the pages while others wait
+#endif
 constexpr int kMaxJobs = 1024;     // job-table entries (jobs in flight or waiting)
 constexpr int kMaxSlices = 4096;   // slices per job
 constexpr int kMaxSide = 2048;     // largest grid side (sets the page size)
@@ -70,7 +76,7 @@ struct Group {
   std::vector<Cohort> cohorts;
   std::vector<ItemDev> host_items;
   DevBuf items_dev;
-  static constexpr int kAhead = 4;    // slice steps a side may run ahead of its device work
+  static constexpr int kAhead = TOTEM_SERVER_AHEAD;  // slice steps a side may run ahead of its device work
   hipEvent_t ahead_ev[kAhead] = {};
   bool ahead_set[kAhead] = {};
   static constexpr int kRing = 3;     // pinned staging buffers of the item list
@@ -162,8 +168,8 @@ void admit(Server& s) {
     Group& g = group_of(s, J.j.n);
     const long held = J.next - J.done;  // probes of J in flight
     long want = J.total - J.next;
-    if (s.order.size() > 1) {
-      const long share = (long)std::max(1, s.n_pages / 2) * g.per_page;
+    if (s.order.size() > 1 && TOTEM_SERVER_SHARE > 1) {
+      const long share = (long)std::max(1, s.n_pages / TOTEM_SERVER_SHARE) * g.per_page;
       want = std::min(want, share - held);
     }
     if (want <= 0) {  // at its share: the jobs behind it go next
