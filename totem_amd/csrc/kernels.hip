@@ -119,7 +119,25 @@ template <> struct PassSpec<768, 1> : PassSpec<768, 0> {
   using RowPlan = PlanB<768>;
   static constexpr int rows_threads = 128;
 };
-template <int N> constexpr int pass_variants() { return (N >= 256 && N != 320) ? 2 : 1; }
+// 1024, variant 2: rows transformed by 8.8.8.2 on 128 lanes (8 elements per lane,
+// fewer registers per wave, one more exchange), two rows per workgroup (2 x 8 tiles).
+template <> struct PassSpec<1024, 2> : PassSpec<1024, 0> {
+  using RowPlan = PlanB<1024>;
+  static constexpr int rows_threads = 256;
+};
+// ... and with the transmission loaded after the inverse transform (59 VGPRs, 8 waves/SIMD).
+template <> struct PassSpec<1024, 3> : PassSpec<1024, 2> {
+  static constexpr bool rows_tprefetch = false;
+};
+template <int N> constexpr int pass_variants() { return N == 1024 ? 4 : (N >= 256 && N != 320) ? 2 : 1; }
+
+// Calls f(std::integral_constant<int, V>) for the compiled variant V == v of grid side S.
+template <int S, int V = 0, class F> static auto on_variant(int v, F&& f) {
+  if constexpr (V + 1 < pass_variants<S>()) {
+    if (v != V) return on_variant<S, V + 1>(v, static_cast<F&&>(f));
+  }
+  return f(std::integral_constant<int, V>{});
+}
 
 template <int N, int V = 0> struct RowCfg {
   using Spec = PassSpec<N, V>;
@@ -847,7 +865,7 @@ template <class... PLS> struct PlanSet {
   }
 };
 using AllPlans = PlanSet<Plan<64>, Plan<128>, Plan<256>, Plan<320>, Plan<512>, Plan<768>, Plan<1024>, Plan<1536>,
-                         Plan<2048>, PlanB<768>>;
+                         Plan<2048>, PlanB<768>, PlanB<1024>>;
 __device__ __align__(16) float2 g_twiddles[AllPlans::total];
 
 template <class PL> void twiddle_host_plan(std::vector<float2>& out) {
@@ -971,24 +989,16 @@ int pass_variant_count(int n) {
 }
 
 std::string pass_variant_desc(int n, int v) {
-#define X(S)                                                          \
-  if (n == S) {                                                       \
-    if constexpr (pass_variants<S>() > 1)                             \
-      if (v == 1) return variant_desc<S, 1>();                        \
-    return variant_desc<S, 0>();                                      \
-  }
+#define X(S) \
+  if (n == S) return on_variant<S>(v, [](auto vc) { return variant_desc<S, decltype(vc)::value>(); });
   TOTEM_FOR_SIZES(X)
 #undef X
   return "";
 }
 
 int detector_groups(int n, int v) {
-#define X(S)                                                          \
-  if (n == S) {                                                       \
-    if constexpr (pass_variants<S>() > 1)                             \
-      if (v == 1) return S / ColCfg<S, 1>::C;                         \
-    return S / ColCfg<S, 0>::C;                                       \
-  }
+#define X(S) \
+  if (n == S) return on_variant<S>(v, [](auto vc) { return S / ColCfg<S, decltype(vc)::value>::C; });
   TOTEM_FOR_SIZES(X)
 #undef X
   return 0;
@@ -1001,22 +1011,20 @@ int detector_groups_max(int n) {
 }
 
 void pass_rows(int n, int v, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step) {
-#define X(S)                                                                    \
-  if (n == S) {                                                                 \
-    if constexpr (pass_variants<S>() > 1)                                       \
-      if (v == 1) return launch_pass_rows<S, 1>(st, items, n_items, jobs, step); \
-    return launch_pass_rows<S, 0>(st, items, n_items, jobs, step);              \
+#define X(S)                                                                                          \
+  if (n == S) {                                                                                       \
+    on_variant<S>(v, [&](auto vc) { launch_pass_rows<S, decltype(vc)::value>(st, items, n_items, jobs, step); }); \
+    return;                                                                                           \
   }
   TOTEM_FOR_SIZES(X)
 #undef X
 }
 
 void pass_cols(int n, int v, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step) {
-#define X(S)                                                                    \
-  if (n == S) {                                                                 \
-    if constexpr (pass_variants<S>() > 1)                                       \
-      if (v == 1) return launch_pass_cols<S, 1>(st, items, n_items, jobs, step); \
-    return launch_pass_cols<S, 0>(st, items, n_items, jobs, step);              \
+#define X(S)                                                                                          \
+  if (n == S) {                                                                                       \
+    on_variant<S>(v, [&](auto vc) { launch_pass_cols<S, decltype(vc)::value>(st, items, n_items, jobs, step); }); \
+    return;                                                                                           \
   }
   TOTEM_FOR_SIZES(X)
 #undef X

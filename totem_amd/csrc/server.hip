@@ -12,10 +12,11 @@
 // Wave slots (wave [N][N] complex64 + FP64 detector partials) live in one arena
 // split into pages of the largest slot size; a page serves one grid side at a
 // time and returns to the shared pool when its last slot is released.  Admission
-// is first come, first served over all grid sides, with one job holding at most
-// half the pages while others wait; when a waiting job with nothing in flight
-// cannot get a single slot, the jobs behind it stop taking new work until pages
-// drain to it, so no job starves.
+// is first come, first served over all grid sides; a job holds at most half the
+// pages while jobs submitted after it started wait (a long job does not stall
+// later arrivals; jobs queued together still run in order); when a waiting job
+// with nothing in flight cannot get a single slot, the jobs behind it stop taking
+// new work until pages drain to it, so no job starves.
 //
 // Results do not depend on what else is in flight: every probe's arithmetic and
 // its detector reduction are the same as in a run of its job alone.
@@ -59,6 +60,7 @@ struct Job {
   long long id;
   int table;
   long total = 0, next = 0, done = 0;
+  long long started = 0;     // next job id when admission first considered the job (0: not yet)
   std::vector<int> mos;
   hipEvent_t ev = nullptr;   // recorded after the job's last reduction
   bool closed = false;
@@ -168,7 +170,11 @@ void admit(Server& s) {
     Group& g = group_of(s, J.j.n);
     const long held = J.next - J.done;  // probes of J in flight
     long want = J.total - J.next;
-    if (s.order.size() > 1 && TOTEM_SERVER_SHARE > 1) {
+    if (J.started == 0) J.started = s.next_id;
+    // jobs submitted after J started (ids only grow, so the last one waiting tells)
+    // hold J to its share of the pages; jobs that were queued before J started wait
+    // their turn, first come first served
+    if (TOTEM_SERVER_SHARE > 1 && s.order.size() > 1 && s.order.back() >= J.started) {
       const long share = (long)std::max(1, s.n_pages / TOTEM_SERVER_SHARE) * g.per_page;
       want = std::min(want, share - held);
     }
