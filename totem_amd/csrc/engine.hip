@@ -755,6 +755,7 @@ static void specimen_transmission_impl(totem_specimen& sp, uint64_t seed, int co
 
 struct StemWorkspace {
   float2* psi;
+  float2* prop;  // the job's propagator in the columns pass's column-group order
   double* partial;
   ItemDev* items;
   JobDev* job;
@@ -769,6 +770,7 @@ static size_t stem_ws_bytes(int n, int S, int A, int L, int batch, long total) {
   const int G = detector_groups_max(n);
   size_t b = 0;
   b += align_up((size_t)batch * n * n * sizeof(float2));
+  b += align_up((size_t)n * n * sizeof(float2));
   b += align_up((size_t)batch * (L + 1) * A * G * sizeof(double));
   b += align_up((size_t)(total + batch) * sizeof(ItemDev));
   b += align_up(sizeof(JobDev));
@@ -783,6 +785,8 @@ static StemWorkspace carve(void* base, int n, int S, int A, int L, int batch, lo
   StemWorkspace w;
   w.psi = reinterpret_cast<float2*>(p);
   p += align_up((size_t)batch * n * n * sizeof(float2));
+  w.prop = reinterpret_cast<float2*>(p);
+  p += align_up((size_t)n * n * sizeof(float2));
   w.partial = reinterpret_cast<double*>(p);
   p += align_up((size_t)batch * (L + 1) * A * G * sizeof(double));
   w.items = reinterpret_cast<ItemDev*>(p);
@@ -915,9 +919,14 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   tw_table(n);
   std::vector<int> mos, hit;
   mid_walk(j, mos, hit);
+  // the propagator regrouped once per run into the columns pass's read order
+  const int pc = pass_cols_width(n, plan);
+  prop_column_groups(st, static_cast<const float2*>(j.prop), w.prop, n, pc);
+  HIP_OK(hipGetLastError());
   JobDev jd;
   jd.t = static_cast<const float2*>(j.t);
-  jd.prop = static_cast<const float2*>(j.prop);
+  jd.prop = w.prop;
+  jd.prop_cols = pc;
   jd.aper = j.aper;
   jd.aper2 = j.aper2;
   jd.mid_of_slice = L > 0 ? w.mid_of_slice : nullptr;

@@ -23,6 +23,8 @@ struct JobDev {
   float2* exit;               // exit-face wave Psi(k) [n_out][N][N] (zero frequency at [0][0]) or null
   int n_out;                  // output columns (probe positions x focal members)
   int wrap;                   // periodic window into the potential
+  int prop_cols;              // C > 0: prop stored column group by column group, element (r, g*C + c)
+                              // at (g*N + r)*C + c (the columns pass's read order); 0: row-major
 };
 
 // One probe position of one job (a "work item") and the device memory it owns

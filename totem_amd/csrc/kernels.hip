@@ -66,6 +66,9 @@ namespace totem {
 
 // ------------------------------------------------------------------------- helpers
 
+#ifndef TOTEM_ROWS_K
+#define TOTEM_ROWS_K 1  // row groups per workgroup (loads of the next one overlap the FFTs of this one)
+#endif
 #ifndef TOTEM_ROWS_TPREFETCH
 #define TOTEM_ROWS_TPREFETCH 1  // load the transmission window together with the wave
 #endif
@@ -101,6 +104,7 @@ template <int N, int V> struct PassSpec {
                                 : N == 2048 ? TOTEM_COLS_C2048 : 8;
   static constexpr bool cols_split = N == 2048 && TOTEM_COLS_SPLIT2048;
   static constexpr bool rows_tprefetch = TOTEM_ROWS_TPREFETCH;  // transmission loaded with the wave
+  static constexpr int rows_k = TOTEM_ROWS_K;                   // row groups per rows workgroup, in turn
 };
 // Variant 1 of most sizes: the rows pass loads its transmission window after the
 // inverse transform, not with the wave (fewer registers live across the
@@ -129,6 +133,9 @@ template <> struct PassSpec<1024, 2> : PassSpec<1024, 0> {
 template <> struct PassSpec<1024, 3> : PassSpec<1024, 2> {
   static constexpr bool rows_tprefetch = false;
 };
+// (Each workgroup taking two row groups in turn, the second group's wave loads
+// issued before the first group's transforms -- rows_k = 2 on this plan, 95 VGPRs
+// -- measured 90.2 against 88.9 us per 26-probe launch and is not compiled.)
 template <int N> constexpr int pass_variants() { return N == 1024 ? 4 : (N >= 256 && N != 320) ? 2 : 1; }
 
 // Calls f(std::integral_constant<int, V>) for the compiled variant V == v of grid side S.
@@ -143,6 +150,7 @@ template <int N, int V = 0> struct RowCfg {
   using Spec = PassSpec<N, V>;
   using PL = typename Spec::RowPlan;
   static constexpr bool TPREFETCH = Spec::rows_tprefetch;
+  static constexpr int K = Spec::rows_k;
   static constexpr int T = PL::T;
   static constexpr int RW = (Spec::rows_threads / T) > 0 ? Spec::rows_threads / T : 1;  // rows per workgroup
   static constexpr int THREADS = RW * T;
@@ -256,9 +264,6 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // ------------------------------------------------------------------------ pass R
 
-#ifndef TOTEM_ROWS_K
-#define TOTEM_ROWS_K 1  // row groups per workgroup (loads of the next one overlap the FFTs of this one)
-#endif
 #ifndef TOTEM_COLS_HPREFETCH
 #define TOTEM_COLS_HPREFETCH 1  // load the propagator before the forward transform
 #endif
@@ -334,7 +339,7 @@ __global__ void __launch_bounds__((RowCfg<N, V>::THREADS), TOTEM_ROWS_MINWAVES)
 k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step, const float2* __restrict__ tw) {
   using Cfg = RowCfg<N, V>;
   using PL = typename Cfg::PL;
-  constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, K = TOTEM_ROWS_K;
+  constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T, K = Cfg::K;
   constexpr int LLP = Cfg::LL + rows_lds_pad<N, V>();
   static_assert(Cfg::NBUF * RW * LLP * sizeof(float2) <= kLdsLimit, "rows-pass LDS above 64 KiB");
   __shared__ float2 lds[Cfg::NBUF * RW * LLP];
@@ -515,14 +520,18 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
       const JobDev& jb = *ua.jb;
       const int col = ua.cg * C + c;
       const auto hr = buf_rsrc(jb.prop, kBytes);
+      // the propagator in column-group order: a wave's 64 lanes read 512 contiguous bytes
+      const bool grouped = jb.prop_cols == C;
+      const int poff = grouped ? ((ua.cg * N + t) * C + c) * (int)sizeof(float2) : hoffa;
+      const int pstep = grouped ? T * C * (int)sizeof(float2) : kStep;
       float2 hv[TOTEM_COLS_HPREFETCH ? E : 1];
       if constexpr (TOTEM_COLS_HPREFETCH) {
 #pragma unroll
-        for (int v = 0; v < E; ++v) hv[v] = bld(hr, hoffa, v * kStep);
+        for (int v = 0; v < E; ++v) hv[v] = bld(hr, poff, v * pstep);
       }
       block_fft<N, -1, Cfg::PP, 0, Cfg::SPLIT, PL>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
-      for (int v = 0; v < E; ++v) xa[v] = cmul(xa[v], TOTEM_COLS_HPREFETCH ? hv[v] : bld(hr, hoffa, v * kStep));
+      for (int v = 0; v < E; ++v) xa[v] = cmul(xa[v], TOTEM_COLS_HPREFETCH ? hv[v] : bld(hr, poff, v * pstep));
       bool keep = true;
       {
         if (ua.mid >= 0) {
@@ -832,6 +841,16 @@ k_accumulate_mfma16(const float2* __restrict__ ex, const float2* __restrict__ ey
   }
 }
 
+// prop [N][N] row-major -> column groups of C columns, rows in order: element
+// (r, g*C + c) at (g*N + r)*C + c.
+__global__ void k_prop_cols(const float2* __restrict__ prop, float2* __restrict__ out, int n, int C) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)n * n) return;
+  const int r = (int)(i / n), col = (int)(i - (long)r * n);
+  const int g = col / C, c = col - g * C;
+  out[((long)g * n + r) * C + c] = prop[i];
+}
+
 __global__ void k_fill_c64(float2* __restrict__ p, long n, float re, float im) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = make_float2(re, im);
@@ -929,7 +948,7 @@ const float2* tw_table(int n) {
 template <int N, int V> void launch_pass_rows(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
                                               int step) {
   using Cfg = RowCfg<N, V>;
-  constexpr int K = TOTEM_ROWS_K;
+  constexpr int K = Cfg::K;
   static_assert((N / Cfg::RW) % K == 0, "row groups per probe must be a multiple of K");
   const dim3 grid((unsigned)(n_items * (N / Cfg::RW) / K));
   hipLaunchKernelGGL((k_pass_rows<N, V>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step,
@@ -1036,6 +1055,19 @@ void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, co
   if (total <= 0) return;
   hipLaunchKernelGGL(k_reduce_det, dim3((total + 255) / 256), dim3(256), 0, st, items, list, n, jobs, groups, lmax1,
                      amax);
+}
+
+int pass_cols_width(int n, int v) {
+#define X(S) \
+  if (n == S) return on_variant<S>(v, [](auto vc) { return ColCfg<S, decltype(vc)::value>::C; });
+  TOTEM_FOR_SIZES(X)
+#undef X
+  return 0;
+}
+
+void prop_column_groups(hipStream_t st, const float2* prop, float2* out, int n, int C) {
+  const long total = (long)n * n;
+  hipLaunchKernelGGL(k_prop_cols, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, prop, out, n, C);
 }
 
 void fill(hipStream_t st, float2* p, long n, float re, float im) {

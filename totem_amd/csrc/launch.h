@@ -27,6 +27,10 @@ int detector_groups_max(int n);           // over the variants (sizes detector p
 
 // One slice step of every listed item (slice = step - item.start_step; items outside
 // [0, S) of their job are skipped), with pass variant `variant` of n.
+// Columns per columns-pass workgroup of plan v of grid side n.
+int pass_cols_width(int n, int v);
+// prop [n][n] row-major -> the column-group order JobDev::prop_cols describes.
+void prop_column_groups(hipStream_t st, const float2* prop, float2* out, int n, int C);
 void pass_rows(int n, int variant, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
 void pass_cols(int n, int variant, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
 // Detector sums of the items items[list[e]] (list may be null: items[e]), e < n, into their

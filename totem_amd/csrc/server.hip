@@ -328,6 +328,7 @@ void submit_impl(Server& s, const totem_stem_job& j, long long* id) {
   jd.L = j.n_mid;
   jd.n_out = (int)J.total;
   jd.wrap = j.wrap;
+  jd.prop_cols = 0;  // the caller's row-major propagator
   // the entry is free (its previous job completed), so a blocking copy cannot race a kernel
   HIP_OK(hipMemcpy(meta, J.mos.data(), J.mos.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(meta + kMaxSlices, hit.data(), hit.size() * sizeof(int), hipMemcpyHostToDevice));
