@@ -136,7 +136,13 @@ template <> struct PassSpec<1024, 3> : PassSpec<1024, 2> {
 // (Each workgroup taking two row groups in turn, the second group's wave loads
 // issued before the first group's transforms -- rows_k = 2 on this plan, 95 VGPRs
 // -- measured 90.2 against 88.9 us per 26-probe launch and is not compiled.)
-template <int N> constexpr int pass_variants() { return N == 1024 ? 4 : (N >= 256 && N != 320) ? 2 : 1; }
+// 512, variant 2: rows transformed by 8.8.8 on 64 lanes (8 elements per lane), four
+// rows per workgroup (4 x 4 tiles, as variant 0).
+template <> struct PassSpec<512, 2> : PassSpec<512, 0> {
+  using RowPlan = PlanB<512>;
+  static constexpr int rows_threads = 256;
+};
+template <int N> constexpr int pass_variants() { return N == 1024 ? 4 : N == 512 ? 3 : (N >= 256 && N != 320) ? 2 : 1; }
 
 // Calls f(std::integral_constant<int, V>) for the compiled variant V == v of grid side S.
 template <int S, int V = 0, class F> static auto on_variant(int v, F&& f) {
@@ -884,7 +890,7 @@ template <class... PLS> struct PlanSet {
   }
 };
 using AllPlans = PlanSet<Plan<64>, Plan<128>, Plan<256>, Plan<320>, Plan<512>, Plan<768>, Plan<1024>, Plan<1536>,
-                         Plan<2048>, PlanB<768>, PlanB<1024>>;
+                         Plan<2048>, PlanB<768>, PlanB<1024>, PlanB<512>>;
 __device__ __align__(16) float2 g_twiddles[AllPlans::total];
 
 template <class PL> void twiddle_host_plan(std::vector<float2>& out) {
