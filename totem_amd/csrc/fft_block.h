@@ -42,6 +42,11 @@ __device__ __forceinline__ float2 bld(__amdgpu_buffer_rsrc_t r, int voff, int so
   const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
   return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
 }
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 bld4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
 __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, int voff, int soff, float2 a) {
   const u32x2 v = {__float_as_uint(a.x), __float_as_uint(a.y)};
   __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
@@ -205,9 +210,10 @@ template <int R, int SIGN> __device__ __forceinline__ void dft(float2* a) {
 // W^(q*k) with k = b mod P and W = exp(SIGN 2 pi i / (P R)), runs DFT_R and
 // writes y[(b - k)*R + k + r*P].  Lane t runs butterflies b = t + T*u.
 //
-// Twiddles: the plan's table holds, for every stage after the first, P blocks
-// of R entries, block k = exp(-2 pi i q k / (P R)) for q < R (entry 0 is 1), so a
-// butterfly reads its R-1 factors contiguously (16-byte loads).
+// Twiddles: the plan's table holds, for every stage after the first, the factors
+// exp(-2 pi i q k / (P R)) for q < R and k < P, q-major (pairs of q for even R), so
+// that one load instruction of a wave, whose lanes hold neighbouring k, reads
+// neighbouring entries (16-byte loads for even R).
 //
 // LDS: with PP (ping-pong) two buffers are used alternately, so one barrier per
 // exchange suffices (the next exchange writes the other buffer, and the one after
@@ -249,21 +255,22 @@ template <int T, int P, int R, int U> struct StageTw {
   float2 w[NW];
   __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
     if constexpr (P > 1) {
+      // buffer loads: the lane's k in the vector offset, the q part in the scalar
+      // offset (no 64-bit address per q)
+      const auto rs = buf_rsrc(tw, (unsigned)(P * R * sizeof(float2)));
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = (t + T * u) % P;
-        const float2* tk = tw + k * R;
         if constexpr (R % 2 == 0) {
-          const float4* t4 = reinterpret_cast<const float4*>(__builtin_assume_aligned(tk, 16));
 #pragma unroll
-          for (int q = 0; q < R / 2; ++q) {
-            const float4 v = t4[q];
-            w[u * R + 2 * q] = make_float2(v.x, v.y);
-            w[u * R + 2 * q + 1] = make_float2(v.z, v.w);
+          for (int j = 0; j < R / 2; ++j) {
+            const float4 v = bld4(rs, k * 16, j * P * 16);
+            w[u * R + 2 * j] = make_float2(v.x, v.y);
+            w[u * R + 2 * j + 1] = make_float2(v.z, v.w);
           }
         } else {
 #pragma unroll
-          for (int q = 1; q < R; ++q) w[u * R + q] = tk[q];
+          for (int q = 1; q < R; ++q) w[u * R + q] = bld(rs, k * 8, q * P * 8);
         }
       }
     }

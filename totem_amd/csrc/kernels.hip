@@ -893,17 +893,26 @@ using AllPlans = PlanSet<Plan<64>, Plan<128>, Plan<256>, Plan<320>, Plan<512>, P
                          Plan<2048>, PlanB<768>, PlanB<1024>, PlanB<512>>;
 __device__ __align__(16) float2 g_twiddles[AllPlans::total];
 
+// Stage (P, R) of a plan, P > 1: the factor exp(-2 pi i q k / (P R)) of input q of a
+// butterfly with k = b mod P.  Laid out q-major so that one load instruction, whose
+// lanes hold neighbouring k, reads neighbouring entries: for even R the pair
+// (q, q + 1) = (2j, 2j + 1) of block k sits at [(j P + k) 2 + {0, 1}] (one 16-byte
+// load per pair), for odd R entry q of block k at [q P + k].
 template <class PL> void twiddle_host_plan(std::vector<float2>& out) {
   out.clear();
   int p = 1;
   for (int i = 0; i < PL::NRAD; ++i) {
     const int r = PL::radix(i);
-    if (p > 1)
+    if (p > 1) {
+      const size_t base = out.size();
+      out.resize(base + (size_t)p * r);
       for (int k = 0; k < p; ++k)
         for (int q = 0; q < r; ++q) {
           const double a = -2.0 * M_PI * (double)q * (double)k / (double)(p * r);
-          out.push_back(unit_float2(std::cos(a), std::sin(a)));
+          const size_t at = r % 2 == 0 ? ((size_t)(q / 2) * p + k) * 2 + q % 2 : (size_t)q * p + k;
+          out[base + at] = unit_float2(std::cos(a), std::sin(a));
         }
+    }
     p *= r;
   }
 }
