@@ -408,6 +408,7 @@ template <int N> struct PlanB;
 template <> struct PlanB<768> : PlanDef<768, 64, 4, 4, 4, 4, 3> {};  // 12 elements per lane
 template <> struct PlanB<1024> : PlanDef<1024, 128, 8, 8, 8, 2> {};  // 8 elements per lane
 template <> struct PlanB<512> : PlanDef<512, 64, 8, 8, 8> {};          // 8 elements per lane
+template <> struct PlanB<1536> : PlanDef<1536, 128, 4, 4, 4, 4, 2, 3> {};  // 12 elements per lane
 
 // Number of twiddle entries of a plan's table.
 template <class PL> __host__ __device__ constexpr int plan_twiddle_len() {

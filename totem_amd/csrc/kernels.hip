@@ -90,7 +90,7 @@ namespace totem {
 // the column exchanges run through half buffers.  Variant 0 is the default; the
 // others are chosen at run time per device (totem_stem_set_plan, or measured by
 // totem_stem_autotune), for a whole run or served grid side at a time.
-template <int N, int V> struct PassSpec {
+template <int N> struct PassSpecBase {
   using RowPlan = Plan<N>;
   using ColPlan = Plan<N>;
   static constexpr int rows_threads = (N == 1024 || N == 768) ? TOTEM_ROWS_THREADS_SMALL
@@ -106,10 +106,14 @@ template <int N, int V> struct PassSpec {
   static constexpr bool rows_tprefetch = TOTEM_ROWS_TPREFETCH;  // transmission loaded with the wave
   static constexpr int rows_k = TOTEM_ROWS_K;                   // row groups per rows workgroup, in turn
 };
+template <int N, int V> struct PassSpec : PassSpecBase<N> {};
+// The plain 2-D FFT kernels (potential build, fft2d) use the base schedule of each
+// side whatever the pass variants specialise.
+constexpr int kBaseSpec = 15;
 // Variant 1 of most sizes: the rows pass loads its transmission window after the
 // inverse transform, not with the wave (fewer registers live across the
 // transform: 1024 points 118 -> 88 VGPRs, the load latency then hidden by more waves).
-template <int N> struct PassSpecLateT : PassSpec<N, 0> {
+template <int N> struct PassSpecLateT : PassSpecBase<N> {
   static constexpr bool rows_tprefetch = false;
 };
 template <> struct PassSpec<256, 1> : PassSpecLateT<256> {};
@@ -118,14 +122,20 @@ template <> struct PassSpec<1024, 1> : PassSpecLateT<1024> {};
 template <> struct PassSpec<1536, 1> : PassSpecLateT<1536> {};
 template <> struct PassSpec<2048, 1> : PassSpecLateT<2048> {};
 // 768: rows transformed by the 12-element plan 4.4.4.4.3 on 64 lanes (one row per
-// wave, 2 x 8 tiles); columns keep the 24-element plan.
-template <> struct PassSpec<768, 1> : PassSpec<768, 0> {
+// wave, 2 x 8 tiles); variant 0 also the columns, 4 per columns workgroup, variant 1
+// the columns by the 24-element plan, variant 2 both passes by the 24-element plan.
+template <> struct PassSpec<768, 1> : PassSpecBase<768> {
   using RowPlan = PlanB<768>;
   static constexpr int rows_threads = 128;
 };
+template <> struct PassSpec<768, 0> : PassSpec<768, 1> {
+  using ColPlan = PlanB<768>;
+  static constexpr int cols_c = 4;
+};
+template <> struct PassSpec<768, 2> : PassSpecBase<768> {};
 // 1024, variant 2: rows transformed by 8.8.8.2 on 128 lanes (8 elements per lane,
 // fewer registers per wave, one more exchange), two rows per workgroup (2 x 8 tiles).
-template <> struct PassSpec<1024, 2> : PassSpec<1024, 0> {
+template <> struct PassSpec<1024, 2> : PassSpecBase<1024> {
   using RowPlan = PlanB<1024>;
   static constexpr int rows_threads = 256;
 };
@@ -138,11 +148,20 @@ template <> struct PassSpec<1024, 3> : PassSpec<1024, 2> {
 // -- measured 90.2 against 88.9 us per 26-probe launch and is not compiled.)
 // 512, variant 2: rows transformed by 8.8.8 on 64 lanes (8 elements per lane), four
 // rows per workgroup (4 x 4 tiles, as variant 0).
-template <> struct PassSpec<512, 2> : PassSpec<512, 0> {
+template <> struct PassSpec<512, 2> : PassSpecBase<512> {
   using RowPlan = PlanB<512>;
   static constexpr int rows_threads = 256;
 };
-template <int N> constexpr int pass_variants() { return N == 1024 ? 4 : N == 512 ? 3 : (N >= 256 && N != 320) ? 2 : 1; }
+// 1536, variant 2: rows by 4.4.4.4.2.3 on 128 lanes (12 elements per lane), two rows
+// per rows workgroup (2 x 8 tiles); columns by the 24-element plan, 4 per workgroup.
+template <> struct PassSpec<1536, 2> : PassSpecBase<1536> {
+  using RowPlan = PlanB<1536>;
+  static constexpr int rows_threads = 256;
+  static constexpr int cols_c = 4;
+};
+template <int N> constexpr int pass_variants() {
+  return N == 1024 ? 4 : (N == 512 || N == 768 || N == 1536) ? 3 : (N >= 256 && N != 320) ? 2 : 1;
+}
 
 // Calls f(std::integral_constant<int, V>) for the compiled variant V == v of grid side S.
 template <int S, int V = 0, class F> static auto on_variant(int v, F&& f) {
@@ -176,6 +195,8 @@ template <int N, int V = 0> struct ColCfg {
   static constexpr int THREADS = T * C;
   static constexpr int LC = SPLIT ? col_lds_len<N / 2, C>() : col_lds_len<N, C>();  // one exchange buffer
 };
+template <int N> using FftRowCfg = RowCfg<N, kBaseSpec>;
+template <int N> using FftColCfg = ColCfg<N, kBaseSpec>;
 // Every pass kernel's static LDS (transform buffers plus the detector's wave sums)
 // stays within 64 KiB per workgroup.
 #ifndef TOTEM_LDS_LIMIT
@@ -602,8 +623,8 @@ __global__ void k_reduce_det(const ItemDev* __restrict__ items, const int* __res
 // --------------------------------------------------------------- plain 2-D FFT passes
 
 template <int N, int SIGN>
-__global__ void __launch_bounds__(RowCfg<N>::THREADS) k_fft_rows(float2* __restrict__ data, long rows, const float2* __restrict__ tw) {
-  using Cfg = RowCfg<N>;
+__global__ void __launch_bounds__(FftRowCfg<N>::THREADS) k_fft_rows(float2* __restrict__ data, long rows, const float2* __restrict__ tw) {
+  using Cfg = FftRowCfg<N>;
   constexpr int T = Cfg::T, RW = Cfg::RW, E = N / T;
   static_assert(RW * Cfg::LL * sizeof(float2) <= kLdsLimit, "row FFT LDS above 64 KiB");
   __shared__ float2 lds[RW * Cfg::LL];
@@ -624,10 +645,10 @@ __global__ void __launch_bounds__(RowCfg<N>::THREADS) k_fft_rows(float2* __restr
 
 // epi: 0 store, 1 store exp(i*sigma*V) (transmission), 2 store scaled by 1/N^2
 template <int N, int SIGN>
-__global__ void __launch_bounds__(ColCfg<N>::THREADS)
+__global__ void __launch_bounds__(FftColCfg<N>::THREADS)
 k_fft_cols(const float2* __restrict__ src, float2* __restrict__ dst, int epi, float sigma,
            const float2* __restrict__ tw) {
-  using Cfg = ColCfg<N>;
+  using Cfg = FftColCfg<N>;
   constexpr int T = Cfg::T, C = Cfg::C, E = N / T, G = N / C;
   static_assert(Cfg::LC * sizeof(float2) <= kLdsLimit, "column FFT LDS above 64 KiB");
   __shared__ float2 lds[Cfg::LC];
@@ -890,7 +911,7 @@ template <class... PLS> struct PlanSet {
   }
 };
 using AllPlans = PlanSet<Plan<64>, Plan<128>, Plan<256>, Plan<320>, Plan<512>, Plan<768>, Plan<1024>, Plan<1536>,
-                         Plan<2048>, PlanB<768>, PlanB<1024>, PlanB<512>>;
+                         Plan<2048>, PlanB<768>, PlanB<1024>, PlanB<512>, PlanB<1536>>;
 __device__ __align__(16) float2 g_twiddles[AllPlans::total];
 
 // Stage (P, R) of a plan, P > 1: the factor exp(-2 pi i q k / (P R)) of input q of a
@@ -980,8 +1001,8 @@ template <int N, int V> void launch_pass_cols(hipStream_t st, const ItemDev* ite
 }
 template <int N> void launch_fft2d(hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi,
                                    float sigma) {
-  using R = RowCfg<N>;
-  using Cc = ColCfg<N>;
+  using R = FftRowCfg<N>;
+  using Cc = FftColCfg<N>;
   const long rows = (long)batch * N;
   const dim3 rgrid((unsigned)((rows + R::RW - 1) / R::RW));
   const float2* tw = tw_for<Plan<N>>();
