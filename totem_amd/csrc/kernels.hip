@@ -154,10 +154,12 @@ template <> struct PassSpec<512, 2> : PassSpecBase<512> {
 };
 // 1536, variant 2: rows by 4.4.4.4.2.3 on 128 lanes (12 elements per lane), two rows
 // per rows workgroup (2 x 8 tiles); columns by the 24-element plan, 4 per workgroup.
+// Per 26-probe launch: rows 102.6, columns 121.6 us against variant 0's 110.1 and
+// 113.7 (the 2 x 8 tiles cost the columns pass what the rows pass gains; four rows
+// per workgroup keep variant 0's tiles but take the rows pass to 136.8 us).
 template <> struct PassSpec<1536, 2> : PassSpecBase<1536> {
   using RowPlan = PlanB<1536>;
   static constexpr int rows_threads = 256;
-  static constexpr int cols_c = 4;
 };
 template <int N> constexpr int pass_variants() {
   return N == 1024 ? 4 : (N == 512 || N == 768 || N == 1536) ? 3 : (N >= 256 && N != 320) ? 2 : 1;
