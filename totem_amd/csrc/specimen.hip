@@ -377,6 +377,7 @@ size_t sort_pairs_temp_bytes(int n) {
 
 unsigned* sort_pairs(hipStream_t st, void* temp, unsigned* keys, unsigned* keys2, unsigned* vals, unsigned* vals2,
                      int n, int bits) {
+  if (n <= 0) return vals;
   const int G = (n + kSortTile - 1) / kSortTile;
   const int m = kRadix * G, nb = (m + kScanChunk - 1) / kScanChunk;
   unsigned* const hist = static_cast<unsigned*>(temp);
