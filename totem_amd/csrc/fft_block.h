@@ -170,26 +170,32 @@ template <int SIGN> __device__ __forceinline__ void dft8(float2* a) {
   }
 }
 
+// 16 = 4 x 4: DFT4 over n1 of a[4 n1 + n2] for each n2, W16^(n2 k1), DFT4 over n2
+// (184 FP32 operations against 196 for two 8-point halves).
 template <int SIGN> __device__ __forceinline__ void dft16(float2* a) {
-  float2 e[8], o[8];
+  float2 y[4][4];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    e[k] = a[2 * k];
-    o[k] = a[2 * k + 1];
+  for (int n2 = 0; n2 < 4; ++n2) {
+    float2 t[4] = {a[n2], a[4 + n2], a[8 + n2], a[12 + n2]};
+    dft4<SIGN>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
   }
-  dft8<SIGN>(e);
-  dft8<SIGN>(o);
-  o[1] = mul_cs<SIGN>(o[1], kC16, kS16);
-  o[2] = mul_w8<SIGN>(o[2]);
-  o[3] = mul_cs<SIGN>(o[3], kS16, kC16);
-  o[4] = mul_si<SIGN>(o[4]);
-  o[5] = mul_cs<SIGN>(o[5], kneg(kS16), kC16);
-  o[6] = mul_w83<SIGN>(o[6]);
-  o[7] = mul_cs<SIGN>(o[7], kneg(kC16), kS16);
+  y[1][1] = mul_cs<SIGN>(y[1][1], kC16, kS16);               // W^1
+  y[1][2] = mul_w8<SIGN>(y[1][2]);                           // W^2
+  y[1][3] = mul_cs<SIGN>(y[1][3], kS16, kC16);               // W^3
+  y[2][1] = mul_w8<SIGN>(y[2][1]);                           // W^2
+  y[2][2] = mul_si<SIGN>(y[2][2]);                           // W^4
+  y[2][3] = mul_w83<SIGN>(y[2][3]);                          // W^6
+  y[3][1] = mul_cs<SIGN>(y[3][1], kS16, kC16);               // W^3
+  y[3][2] = mul_w83<SIGN>(y[3][2]);                          // W^6
+  y[3][3] = mul_cs<SIGN>(y[3][3], kneg(kC16), kneg(kS16));   // W^9
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    a[k] = cadd(e[k], o[k]);
-    a[k + 8] = csub(e[k], o[k]);
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    dft4<SIGN>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = t[k2];
   }
 }
 
