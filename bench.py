@@ -476,10 +476,14 @@ def main():
         plain_value = M * pp * S / plain_wall
         plain_match = bool(np.array_equal(det_p.cpu().numpy()[0], det_ref[:, :M * pp]))
 
-    # roofline of the dominant kernel (HIP events on the launch stream)
+    # roofline of the dominant kernel (HIP events on the launch stream).  The passes move the
+    # wave's band columns only (the columns where the propagator has a nonzero entry, widened
+    # to whole column groups): f = band / N of each probe's wave is read and written.
     n2 = float(N) * N
-    bytes_rows = batch * 16.0 * n2 + 8.0 * NP * NP          # psi read+write + the shared t slice
-    bytes_cols = batch * 16.0 * n2 + 8.0 * n2                # psi read+write + the propagator
+    _, band = engine.stem_band(optics.unit_complex64(inp.psf), engine.plan_columns(N, engine.plan(N)[0]))
+    f = band / float(N)
+    bytes_rows = batch * 16.0 * f * n2 + 8.0 * NP * NP      # band of psi read+write + the shared t slice
+    bytes_cols = batch * 16.0 * f * n2 + 8.0 * f * n2       # band of psi read+write + its propagator columns
     rows_avg = rows_ms / max(rows_n, 1)
     cols_avg = cols_ms / max(cols_n, 1)
     if rows_avg >= cols_avg:
@@ -522,6 +526,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": kavg,
+                     "band_columns": band, "band_fraction": f,
                      "rows_avg_ms": rows_avg, "cols_avg_ms": cols_avg},
         "roofline_other_pass": {"kernel": oname, "achieved": oach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": (oach / HBM_PEAK_GBS) if oach else None,

@@ -167,6 +167,20 @@ int totem_stem_autotune(int n, int slices, int *best_batch, double *probe_slices
 int totem_stem_plan_count(int n);  /* 0 if n has no plan */
 int totem_stem_plan(int n, int *plan, char *desc, size_t cap);  /* plan in use on the current device */
 int totem_stem_set_plan(int n, int plan);
+/* Columns of the columns pass's workgroups under plan `plan` of n-point grids (0 if
+ * there is no such plan). */
+int totem_stem_plan_columns(int n, int plan);
+/* Band of a run: totem_stem_run flags the columns of job->prop that hold a nonzero
+ * entry and processes only the shortest wrapped run of `group`-column groups
+ * [lo, lo + len) (mod n) covering them (group = totem_stem_plan_columns of the plan
+ * in use; len = 0 if prop is zero, n if no group is empty), extended at its end to a
+ * multiple of 8 groups (at most n) for the columns pass's XCD order.  The wave in every other
+ * column is zero after each propagator multiply, so neither pass reads or writes
+ * it; the detectors and the exit wave see exact zeros there, as with the full
+ * transform.  This is that rule on the host, from per-column flags (nonzero:
+ * the column holds a nonzero entry).  No counterpart in the reference, whose cuFFT
+ * transforms transform every column (cuda_source_lobato.cu:1324-1327, 1388-1391). */
+int totem_stem_band(const int *col_flags, int n, int group, int *lo, int *len);
 /* Device bytes of workspace totem_stem_run needs for `batch` probes in flight. */
 size_t totem_stem_workspace_bytes(const totem_stem_job *job, int batch);
 /* Runs all M*D probes of the job, `batch` at a time (batch = 1 is the plain

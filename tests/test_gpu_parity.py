@@ -286,6 +286,42 @@ def test_stem_mixed_radix_grids_match_oracle(engine, flag, n, np_side, S):
     _check_images(mid_mat, mid_o, "mid-slice detectors")
 
 
+@pytest.mark.parametrize("n,shift", [(128, 64), (128, 37), (768, 300)])
+def test_stem_with_shifted_propagator_band_matches_oracle(engine, n, shift):
+    """A propagator whose nonzero columns form a band that does (shift 37) or does not (shift
+    n/2) wrap through column 0: the passes process that band only (the rest of the wave is zero
+    after every propagator multiply) and the images match the oracle run with the same propagator."""
+    import dataclasses
+    print("case shifted band n=%d shift=%d" % (n, shift), flush=True)
+    cfg, inp = _stem_case("l", M=1, mid=(2,), n=n, np_side=256 if n <= 128 else 1024, cells=2 if n > 128 else 4,
+                          scan=(2, 1), step=7, begin=(11, 23))
+    inp = dataclasses.replace(inp, psf=np.roll(inp.psf, shift, axis=1))
+    lo, ln = engine.stem_band(inp.psf, engine.plan_columns(n, engine.plan(n)[0]))
+    assert 0 < ln < n
+    myresul, mid_mat, _, _ = stem.run(inp)
+    res_o, mid_o = _oracle_stem(inp)
+    # a shifted band can leave a detector without signal (the oracle's FP64 roundoff there is
+    # ~1e-35): relative error per pixel with a floor of 1e-6 of the image's peak
+    for res, ref, what in ((myresul, res_o, "exit-face detectors"), (mid_mat, mid_o, "mid-slice detectors")):
+        rel = np.abs(res - ref) / (np.abs(ref) + 1e-6 * np.abs(ref).max())
+        print("  %s: max rel per pixel %.3e" % (what, rel.max()))
+        assert rel.max() <= 1e-4
+
+
+def test_zero_propagator_gives_zero_images(engine):
+    """A propagator without a nonzero entry: an empty band, every detector image exactly zero (as
+    the oracle's)."""
+    import dataclasses
+    print("case zero propagator", flush=True)
+    cfg, inp = _stem_case("l", M=1, mid=(2,), scan=(2, 2), step=5)
+    inp = dataclasses.replace(inp, psf=np.zeros_like(inp.psf))
+    assert engine.stem_band(inp.psf, 4) == (0, 0)
+    myresul, mid_mat, _, _ = stem.run(inp)
+    res_o, mid_o = _oracle_stem(inp)
+    assert np.all(res_o == 0) and np.all(mid_o == 0)
+    assert np.all(myresul == 0) and np.all(mid_mat == 0)
+
+
 def test_empty_slices_are_vacuum(engine):
     """Slices without atoms (here the last four of six) transmit exactly: t = 1 + 0i, and the scan
     images still match the oracle."""

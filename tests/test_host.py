@@ -300,3 +300,50 @@ def test_planner_keeps_rows_whose_gaussians_do_not_repeat():
     st = _plan(inp)
     assert not st["split"]
     assert st["classes"] == distinct_rows and st["terms"] == len(inp.ele_n)
+
+
+def _band_brute(flags, group):
+    """Shortest length of a wrapped run of whole groups covering every flagged column."""
+    n = len(flags)
+    G = n // group
+    on = [any(flags[g * group:(g + 1) * group]) for g in range(G)]
+    if not any(on):
+        return 0
+    best = G
+    for s in range(G):
+        for ln in range(1, G + 1):
+            if all(not on[g] or (g - s) % G < ln for g in range(G)):
+                best = min(best, ln)
+                break
+    return best * group
+
+
+def test_stem_band_is_the_shortest_wrapped_run_of_groups():
+    """totem_stem_band (the columns a run processes): aligned to the group width, covers every
+    column of prop holding a nonzero entry, and no shorter wrapped run of groups does; a zero
+    propagator gives an empty band and one without an empty group the whole grid."""
+    from totem_amd import engine
+    rng = np.random.default_rng(11)
+    n, group = 64, 4
+    cases = [np.zeros(n, int), np.ones(n, int)]
+    for _ in range(60):
+        f = np.zeros(n, int)
+        k = rng.integers(1, 5)
+        for _ in range(k):  # a few wrapped runs of flagged columns
+            a, ln = rng.integers(0, n), rng.integers(1, 24)
+            f[(a + np.arange(ln)) % n] = 1
+        cases.append(f)
+    for f in cases:
+        prop = np.zeros((3, n), np.complex64)
+        prop[1, f.astype(bool)] = 0.5 - 0.25j
+        lo, ln = engine.stem_band(prop, group)
+        assert lo % group == 0 and ln % group == 0 and 0 <= lo < n and 0 <= ln <= n
+        for c in np.flatnonzero(f):
+            assert (c - lo) % n < ln, (f, lo, ln, c)
+        assert ln == _band_brute(list(f), group), (f, lo, ln)
+    # the propagator of a 2/3 band-limited grid: DC at column 0, the band wraps through it
+    from totem_amd import optics
+    psf = optics.propagator(96, 0.1, 200, 1.9525)
+    lo, ln = engine.stem_band(psf, 4)
+    cols = np.flatnonzero(np.abs(psf).max(axis=0) > 0)
+    assert 0 < ln < 96 and lo > 0 and all((c - lo) % 96 < ln for c in cols)

@@ -761,6 +761,7 @@ struct StemWorkspace {
   JobDev* job;
   int* mid_of_slice;
   int* layer_hit;
+  int* col_flags;  // [n]: column of the propagator holds a nonzero entry
 };
 
 // Workspace of `batch` probes in flight out of `total`: waves and detector
@@ -776,6 +777,7 @@ static size_t stem_ws_bytes(int n, int S, int A, int L, int batch, long total) {
   b += align_up(sizeof(JobDev));
   b += align_up((size_t)std::max(S, 1) * sizeof(int));
   b += align_up((size_t)std::max(L, 1) * sizeof(int));
+  b += align_up((size_t)n * sizeof(int));
   return b;
 }
 
@@ -796,6 +798,8 @@ static StemWorkspace carve(void* base, int n, int S, int A, int L, int batch, lo
   w.mid_of_slice = reinterpret_cast<int*>(p);
   p += align_up((size_t)std::max(S, 1) * sizeof(int));
   w.layer_hit = reinterpret_cast<int*>(p);
+  p += align_up((size_t)std::max(L, 1) * sizeof(int));
+  w.col_flags = reinterpret_cast<int*>(p);
   return w;
 }
 
@@ -891,6 +895,41 @@ static void set_plan(int n, int plan) {
   g_plan[{dev, n}] = plan;
 }
 
+// The shortest wrapped run of C-column groups [lo, lo + len) (mod n) that covers
+// every flagged column: the complement of the longest wrapped run of groups without
+// a flag (the first such run if several tie).  No flag: len = 0; no empty group:
+// the whole grid.
+static void column_band(const int* flags, int n, int C, int& lo, int& len) {
+  const int G = n / C;
+  std::vector<char> on(G, 0);
+  for (int c = 0; c < n; ++c)
+    if (flags[c]) on[c / C] = 1;
+  int n_on = 0;
+  for (int g = 0; g < G; ++g) n_on += on[g];
+  if (n_on == 0) {
+    lo = 0;
+    len = 0;
+    return;
+  }
+  if (n_on == G) {
+    lo = 0;
+    len = n;
+    return;
+  }
+  int best_start = 0, best = 0;
+  for (int g = 0; g < G; ++g) {
+    if (on[g] || on[(g + G - 1) % G] == 0) continue;  // runs start after a flagged group
+    int r = 0;
+    while (r < G && !on[(g + r) % G]) ++r;
+    if (r > best) {
+      best = r;
+      best_start = g;
+    }
+  }
+  lo = ((best_start + best) % G) * C;
+  len = (G - best) * C;
+}
+
 // Per-kernel durations of the last timed run (TOTEM_RUN_TIMING), from HIP events
 // recorded on the launch stream around every rows/columns pass.
 struct KernelTimes {
@@ -919,11 +958,24 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   tw_table(n);
   std::vector<int> mos, hit;
   mid_walk(j, mos, hit);
-  // the propagator regrouped once per run into the columns pass's read order
+  // the propagator regrouped once per run into the columns pass's read order, and
+  // the columns it leaves nonzero
   const int pc = pass_cols_width(n, plan);
   prop_column_groups(st, static_cast<const float2*>(j.prop), w.prop, n, pc);
+  column_flags(st, static_cast<const float2*>(j.prop), w.col_flags, n);
   HIP_OK(hipGetLastError());
+  std::vector<int> col_nz(n);
+  HIP_OK(hipMemcpyAsync(col_nz.data(), w.col_flags, n * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
   JobDev jd;
+  column_band(col_nz.data(), n, pc, jd.band_lo, jd.band_len);
+  // the columns pass runs the band's groups, rounded up to a multiple of 8 so that
+  // its XCD order applies (the added groups hold zeros and are transformed as such)
+  int band_groups = jd.band_len / pc;
+  if (band_groups < n / pc) {
+    band_groups = std::min(n / pc, (band_groups + 7) / 8 * 8);
+    jd.band_len = band_groups * pc;
+  }
   jd.t = static_cast<const float2*>(j.t);
   jd.prop = w.prop;
   jd.prop_cols = pc;
@@ -942,6 +994,12 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   HIP_OK(hipMemcpyAsync(w.job, &jd, sizeof(jd), hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(w.mid_of_slice, mos.data(), S * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(w.layer_hit, hit.data(), hit.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  // detector partials of the groups outside the band and the exit wave's columns
+  // there are never written by the passes: they start (and stay) zero
+  if (jd.band_len < n) {
+    fill(st, w.partial, (long)batch * (L + 1) * A * Gslot, 0.0);
+    if (j.exit_wave) fill(st, static_cast<float2*>(j.exit_wave), total * n * n, 0.f, 0.f);
+  }
   HIP_OK(hipStreamSynchronize(st));  // host staging buffers above are locals
   // every probe's work item, uploaded once: chunk c runs items [c*batch, ...) in
   // wave slots 0..batch-1, which consecutive chunks reuse in stream order
@@ -965,7 +1023,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   auto record = [&](const ItemDev* it, int cnt) {
     for (int s = 0; s < S; ++s) {
       pass_rows(n, plan, st, it, cnt, w.job, s);
-      pass_cols(n, plan, st, it, cnt, w.job, s);
+      if (band_groups > 0) pass_cols(n, plan, st, it, cnt, w.job, s, band_groups);
     }
     reduce_det(st, it, nullptr, cnt, w.job, G, L + 1, A);
   };
@@ -984,7 +1042,7 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
       for (int s = 0; s < S; ++s) {
         pass_rows(n, plan, st, ci, cnt, w.job, s);
         HIP_OK(hipEventRecord(ev[2 * s + 1], st));
-        pass_cols(n, plan, st, ci, cnt, w.job, s);
+        if (band_groups > 0) pass_cols(n, plan, st, ci, cnt, w.job, s, band_groups);
         HIP_OK(hipEventRecord(ev[2 * s + 2], st));
       }
       reduce_det(st, ci, nullptr, cnt, w.job, G, L + 1, A);
@@ -1467,6 +1525,19 @@ int totem_stem_plan(int n, int* plan, char* desc, size_t cap) {
 
 int totem_stem_set_plan(int n, int plan) {
   return guarded([&] { totem::set_plan(n, plan); });
+}
+
+int totem_stem_band(const int* col_flags, int n, int group, int* lo, int* len) {
+  return guarded([&] {
+    require(col_flags && lo && len, "col_flags, lo and len are required");
+    require(n > 0 && group > 0 && n % group == 0, "n must be a positive multiple of group");
+    totem::column_band(col_flags, n, group, *lo, *len);
+  });
+}
+
+int totem_stem_plan_columns(int n, int plan) {
+  if (!size_supported(n) || plan < 0 || plan >= pass_variant_count(n)) return 0;
+  return pass_cols_width(n, plan);
 }
 
 size_t totem_stem_workspace_bytes(const totem_stem_job* job, int batch) {

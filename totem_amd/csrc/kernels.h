@@ -25,6 +25,13 @@ struct JobDev {
   int wrap;                   // periodic window into the potential
   int prop_cols;              // C > 0: prop stored column group by column group, element (r, g*C + c)
                               // at (g*N + r)*C + c (the columns pass's read order); 0: row-major
+  // Columns c of the reciprocal grid with ((c - band_lo) mod N) < band_len: every
+  // column outside holds zeros only in prop, so the wave there is zero after every
+  // propagator multiply and neither pass reads or writes it (slice 0's rows pass
+  // still reads the whole probe; the columns pass is launched for the band's groups
+  // only).  band_lo and band_len are multiples of the columns pass's group width.
+  int band_lo = 0;
+  int band_len = 1 << 30;
 };
 
 // One probe position of one job (a "work item") and the device memory it owns

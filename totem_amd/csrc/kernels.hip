@@ -330,7 +330,23 @@ struct RowUnit {
   __amdgpu_buffer_rsrc_t trs;  // the transmission slice
   int t_off;                   // bytes: the lane's row in it
   int c0, np;
+  int lo, len;                 // the job's column band
 };
+
+// A lane offset beyond every buffer resource's range: buffer loads there return 0
+// and stores are dropped, without a memory access.
+constexpr int kNoAccess = (int)0x80000000u;
+// off if column c0 + T*v lies in a job's band [lo, lo + len) (mod N) (base = c0 - lo),
+// else kNoAccess
+template <int N, int T> __device__ __forceinline__ int band_off(int base, int v, int len, int off) {
+  int d = base + T * v;
+  if constexpr ((N & (N - 1)) == 0) {
+    d &= N - 1;
+  } else {
+    d = d < 0 ? d + N : d;
+  }
+  return d < len ? off : kNoAccess;
+}
 
 template <int N, int V, int K>
 __device__ __forceinline__ RowUnit row_unit(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs,
@@ -360,6 +376,8 @@ __device__ __forceinline__ RowUnit row_unit(const ItemDev* __restrict__ items, c
   r.trs = buf_rsrc(jb.t + (size_t)(r.active ? r.slice : 0) * np2, (unsigned)(np2 * sizeof(float2)));
   r.t_off = pr * r.np * (int)sizeof(float2);
   r.c0 = it.c0;
+  r.lo = jb.band_lo;
+  r.len = jb.band_len;
   return r;
 }
 
@@ -379,9 +397,16 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   constexpr int DSTEP = WaveLay<N, V>::row_step(T) * (int)sizeof(float2);
   float2 xa[E], xb[E];
   RowUnit ua = row_unit<N, V, K>(items, jobs, step, 0, lr, t);
+  // After slice 0 the wave is zero outside the job's band and only its band columns are read.
   if (ua.active) {
+    if (ua.slice == 0) {
 #pragma unroll
-    for (int v = 0; v < E; ++v) xa[v] = bld(ua.src, ua.src_off, ua.src_step * v);
+      for (int v = 0; v < E; ++v) xa[v] = bld(ua.src, ua.src_off, ua.src_step * v);
+    } else {
+#pragma unroll
+      for (int v = 0; v < E; ++v)
+        xa[v] = bld(ua.src, band_off<N, T>(t - ua.lo, v, ua.len, ua.src_off), ua.src_step * v);
+    }
   }
 #pragma unroll
   for (int u = 0; u < K; ++u) {
@@ -390,7 +415,9 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
       ub = row_unit<N, V, K>(items, jobs, step, u + 1, lr, t);
       if (ub.active) {
 #pragma unroll
-        for (int v = 0; v < E; ++v) xb[v] = bld(ub.src, ub.src_off, ub.src_step * v);
+        for (int v = 0; v < E; ++v)
+          xb[v] = bld(ub.src, ub.slice == 0 ? ub.src_off : band_off<N, T>(t - ub.lo, v, ub.len, ub.src_off),
+                      ub.src_step * v);
       }
     }
     if (ua.active) {  // uniform over the workgroup
@@ -421,7 +448,8 @@ k_pass_rows(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
       }
       block_fft<N, -1, Cfg::PP, next_start<N, Cfg::PP, PL>(0), Cfg::SPLIT, PL>(xa, lds, lds1, lay, t, tw);
 #pragma unroll
-      for (int v = 0; v < E; ++v) bst(ua.dst, ua.dst_off, DSTEP * v, xa[v]);
+      for (int v = 0; v < E; ++v)  // the columns pass reads band columns only
+        bst(ua.dst, band_off<N, T>(t - ua.lo, v, ua.len, ua.dst_off), DSTEP * v, xa[v]);
       if (u + 1 < K) __syncthreads();  // LDS reuse by the next unit
     }
     if (u + 1 < K) {
@@ -480,14 +508,18 @@ struct ColUnit {
 
 template <int N, int V, int K>
 __device__ __forceinline__ ColUnit col_unit(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs,
-                                            int step, int u) {
+                                            int step, int u, int groups) {
   using Cfg = ColCfg<N, V>;
   constexpr int G = N / Cfg::C;
   ColUnit r;
-  int item;
-  unit_map<N, K>(blockIdx.x, u, G, item, r.cg);
+  int item, gi;
+  // `groups` groups per item from the first group of the job's band on (all G groups
+  // with the default band)
+  unit_map<N, K>(blockIdx.x, u, groups, item, gi);
   const ItemDev it = items[item];
   r.jb = &jobs[it.job];
+  r.cg = gi + r.jb->band_lo / Cfg::C;
+  if (r.cg >= G) r.cg -= G;
   r.slice = step - it.start_step;
   r.active = r.slice >= 0 && r.slice < r.jb->S;
   r.wr = buf_rsrc(it.wave, (unsigned)N * N * sizeof(float2));
@@ -504,7 +536,7 @@ __device__ __forceinline__ ColUnit col_unit(const ItemDev* __restrict__ items, c
 // at 1024 points 160 vs 128 VGPRs, 101 vs 89 us per launch of 26 probes).
 template <int N, int V>
 __global__ void __launch_bounds__((ColCfg<N, V>::THREADS), TOTEM_COLS_MINWAVES)
-k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step,
+k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, int step, int groups,
             const float2* __restrict__ tw) {
   using Cfg = ColCfg<N, V>;
   using PL = typename Cfg::PL;
@@ -523,7 +555,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
   constexpr unsigned kBytes = (unsigned)N * N * sizeof(float2);
   constexpr int kStep = T * N * (int)sizeof(float2);
   float2 xa[E], xb[E];
-  ColUnit ua = col_unit<N, V, K>(items, jobs, step, 0);
+  ColUnit ua = col_unit<N, V, K>(items, jobs, step, 0, groups);
   // byte offsets of (row t, column): in the wave (WaveLay) and in row-major arrays (H, exit wave);
   // both advance by T*N elements per v
   int voffa = Lay::at(t, ua.cg * C + c) * (int)sizeof(float2);
@@ -537,7 +569,7 @@ k_pass_cols(const ItemDev* __restrict__ items, const JobDev* __restrict__ jobs, 
     ColUnit ub;
     int voffb = 0, hoffb = 0;
     if (u + 1 < K) {
-      ub = col_unit<N, V, K>(items, jobs, step, u + 1);
+      ub = col_unit<N, V, K>(items, jobs, step, u + 1, groups);
       voffb = Lay::at(t, ub.cg * C + c) * (int)sizeof(float2);
       hoffb = (t * N + ub.cg * C + c) * (int)sizeof(float2);
       if (ub.active) {
@@ -880,6 +912,18 @@ __global__ void k_prop_cols(const float2* __restrict__ prop, float2* __restrict_
   out[((long)g * n + r) * C + c] = prop[i];
 }
 
+// One thread per column; consecutive threads read consecutive entries of a row.
+__global__ void k_column_flags(const float2* __restrict__ prop, int* __restrict__ flags, int n) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  int nz = 0;
+  for (int r = 0; r < n && !nz; ++r) {
+    const float2 h = prop[(size_t)r * n + c];
+    nz = h.x != 0.f || h.y != 0.f;
+  }
+  flags[c] = nz;
+}
+
 __global__ void k_fill_c64(float2* __restrict__ p, long n, float re, float im) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = make_float2(re, im);
@@ -993,12 +1037,13 @@ template <int N, int V> void launch_pass_rows(hipStream_t st, const ItemDev* ite
                      tw_for<typename Cfg::PL>());
 }
 template <int N, int V> void launch_pass_cols(hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs,
-                                              int step) {
+                                              int step, int groups) {
   using Cfg = ColCfg<N, V>;
   constexpr int K = TOTEM_COLS_K;
   static_assert((N / Cfg::C) % K == 0, "column groups per probe must be a multiple of K");
-  const dim3 grid((unsigned)(n_items * (N / Cfg::C) / K));
-  hipLaunchKernelGGL((k_pass_cols<N, V>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step,
+  if (groups <= 0 || groups > N / Cfg::C || groups % K != 0) throw std::runtime_error("columns pass: bad group count");
+  const dim3 grid((unsigned)(n_items * groups / K));
+  hipLaunchKernelGGL((k_pass_cols<N, V>), grid, dim3(Cfg::THREADS), 0, st, items, jobs, step, groups,
                      tw_for<typename Cfg::PL>());
 }
 template <int N> void launch_fft2d(hipStream_t st, float2* data, float2* dst, int batch, int sign, int epi,
@@ -1077,10 +1122,13 @@ void pass_rows(int n, int v, hipStream_t st, const ItemDev* items, int n_items, 
 #undef X
 }
 
-void pass_cols(int n, int v, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step) {
+void pass_cols(int n, int v, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step,
+               int groups) {
 #define X(S)                                                                                          \
   if (n == S) {                                                                                       \
-    on_variant<S>(v, [&](auto vc) { launch_pass_cols<S, decltype(vc)::value>(st, items, n_items, jobs, step); }); \
+    on_variant<S>(v, [&](auto vc) {                                                                   \
+      launch_pass_cols<S, decltype(vc)::value>(st, items, n_items, jobs, step, groups);              \
+    });                                                                                               \
     return;                                                                                           \
   }
   TOTEM_FOR_SIZES(X)
@@ -1106,6 +1154,10 @@ int pass_cols_width(int n, int v) {
 void prop_column_groups(hipStream_t st, const float2* prop, float2* out, int n, int C) {
   const long total = (long)n * n;
   hipLaunchKernelGGL(k_prop_cols, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, prop, out, n, C);
+}
+
+void column_flags(hipStream_t st, const float2* prop, int* flags, int n) {
+  hipLaunchKernelGGL(k_column_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, prop, flags, n);
 }
 
 void fill(hipStream_t st, float2* p, long n, float re, float im) {

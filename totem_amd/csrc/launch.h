@@ -31,8 +31,13 @@ int detector_groups_max(int n);           // over the variants (sizes detector p
 int pass_cols_width(int n, int v);
 // prop [n][n] row-major -> the column-group order JobDev::prop_cols describes.
 void prop_column_groups(hipStream_t st, const float2* prop, float2* out, int n, int C);
+// flags[c] = 1 if column c of prop [n][n] (row-major) holds a nonzero entry, else 0.
+void column_flags(hipStream_t st, const float2* prop, int* flags, int n);
 void pass_rows(int n, int variant, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
-void pass_cols(int n, int variant, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
+// The columns pass runs `groups` column groups per item, from the first group of each
+// job's band on (all n / pass_cols_width(n, variant) of them for the default band).
+void pass_cols(int n, int variant, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step,
+               int groups);
 // Detector sums of the items items[list[e]] (list may be null: items[e]), e < n, into their
 // job's det; lmax1 and amax bound (L + 1) and A over the jobs involved.
 void reduce_det(hipStream_t st, const ItemDev* items, const int* list, int n, const JobDev* jobs, int groups,

@@ -270,7 +270,7 @@ void step_group(Server& s, Group& g) {
   const ItemDev* items = g.items_dev.as<ItemDev>();
   const JobDev* jobs = s.jobs_dev.as<JobDev>();
   pass_rows(g.n, g.plan, g.st, items, cnt, jobs, g.step);
-  pass_cols(g.n, g.plan, g.st, items, cnt, jobs, g.step);
+  pass_cols(g.n, g.plan, g.st, items, cnt, jobs, g.step, g.n / pass_cols_width(g.n, g.plan));
   bool removed = false;
   for (Cohort& c : g.cohorts) {
     Job& J = s.jobs.at(c.job);

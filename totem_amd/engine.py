@@ -37,6 +37,7 @@ EXPORTS = (
     "totem_potential_plan_stats",
     "totem_stem_workspace_bytes", "totem_stem_run", "totem_stem_run_static", "totem_kernel_times", "totem_fft2d",
     "totem_stem_preferred_batch", "totem_stem_autotune", "totem_stem_plan_count", "totem_stem_plan",
+    "totem_stem_plan_columns", "totem_stem_band",
     "totem_stem_set_plan", "totem_hrtem_images", "totem_tcc_image", "totem_weighted_pattern",
     "totem_server_create", "totem_server_submit", "totem_server_step", "totem_server_poll", "totem_server_sync",
     "totem_server_destroy",
@@ -154,6 +155,9 @@ def lib():
         L.totem_stem_plan.argtypes = [_I, ctypes.POINTER(_I), ctypes.c_char_p, ctypes.c_size_t]
         L.totem_stem_plan.restype = _I
         L.totem_stem_set_plan.argtypes = [_I, _I]
+        L.totem_stem_plan_columns.argtypes = [_I, _I]
+        L.totem_stem_plan_columns.restype = _I
+        L.totem_stem_band.argtypes = [ctypes.POINTER(_I), _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]
         L.totem_stem_set_plan.restype = _I
         L.totem_specimen_create.argtypes = [ctypes.POINTER(SpecimenDesc), ctypes.POINTER(_VP)]
         L.totem_specimen_transmission.argtypes = [_VP, ctypes.c_ulonglong, _I, _VP, _VP]
@@ -458,6 +462,25 @@ def plan(n):
 def set_plan(n, index):
     """Use pass plan ``index`` for n x n grids on the current device (runs started afterwards)."""
     check(lib().totem_stem_set_plan(int(n), int(index)))
+
+
+def plan_columns(n, index):
+    """Columns per columns-pass workgroup of pass plan ``index`` of n x n grids."""
+    return int(lib().totem_stem_plan_columns(int(n), int(index)))
+
+
+def stem_band(prop, group):
+    """(lo, len): the wrapped run of columns [lo, lo + len) (mod n) a run of the propagator
+    ``prop`` [n, n] processes with ``group`` columns per columns-pass workgroup (the columns
+    holding a nonzero entry, widened to whole groups and to the shortest wrapped run covering
+    them)."""
+    prop = np.asarray(prop)
+    n = prop.shape[-1]
+    flags = np.ascontiguousarray((prop != 0).any(axis=0).astype(np.int32))
+    lo, ln = _I(), _I()
+    check(lib().totem_stem_band(flags.ctypes.data_as(ctypes.POINTER(_I)), int(n), int(group), ctypes.byref(lo),
+                                ctypes.byref(ln)))
+    return lo.value, ln.value
 
 
 def stem_workspace_bytes(job, batch):
