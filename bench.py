@@ -158,7 +158,9 @@ def run_precision_stress(scan):
     out = {"workload": "Si [110] ~100x100x300 A + 2x50 A amorphous C, %d atoms, Lobato, 200 kV 22 mrad, "
                        "ADF 60-200 mrad" % len(cfg.atoms),
            "grid": N, "slices": S, "probes": M * D, "probes_per_launch": batch,
-           "probe_slices_per_s": M * D * S / wall, "wall_s": wall, "transmission_build_s": build_s}
+           "probe_slices_per_s": M * D * S / wall, "wall_s": wall, "transmission_build_s": build_s,
+           "band_columns": engine.stem_band(optics.unit_complex64(psf),
+                                            engine.plan_columns(N, engine.plan(N)[0]))[1]}
     del t, ws, probes, det
     torch.cuda.empty_cache()
     return out

@@ -899,7 +899,7 @@ static void set_plan(int n, int plan) {
 // every flagged column: the complement of the longest wrapped run of groups without
 // a flag (the first such run if several tie).  No flag: len = 0; no empty group:
 // the whole grid.
-static void column_band(const int* flags, int n, int C, int& lo, int& len) {
+void column_band(const int* flags, int n, int C, int& lo, int& len) {
   const int G = n / C;
   std::vector<char> on(G, 0);
   for (int c = 0; c < n; ++c)
@@ -928,6 +928,31 @@ static void column_band(const int* flags, int n, int C, int& lo, int& len) {
   }
   lo = ((best_start + best) % G) * C;
   len = (G - best) * C;
+}
+
+void set_job_band(JobDev& jd, const int* flags, int n, int C) {
+  int lo = 0, len = 0;
+  column_band(flags, n, C, lo, len);
+  // rounded up to a multiple of 8 groups so that the columns pass's XCD order applies
+  // (the added groups hold zeros and are transformed as such)
+  int ng = len / C;
+  if (ng < n / C) {
+    ng = std::min(n / C, (ng + 7) / 8 * 8);
+    len = ng * C;
+  }
+  jd.band_lo = lo;
+  jd.band_len = len;
+  jd.band_g0 = lo / C;
+  jd.band_ng = ng;
+}
+
+void job_band(hipStream_t st, JobDev& jd, const float2* prop, int* flags, int n, int C) {
+  column_flags(st, prop, flags, n);
+  HIP_OK(hipGetLastError());
+  std::vector<int> h(n);
+  HIP_OK(hipMemcpyAsync(h.data(), flags, n * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  set_job_band(jd, h.data(), n, C);
 }
 
 // Per-kernel durations of the last timed run (TOTEM_RUN_TIMING), from HIP events
@@ -962,20 +987,9 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   // the columns it leaves nonzero
   const int pc = pass_cols_width(n, plan);
   prop_column_groups(st, static_cast<const float2*>(j.prop), w.prop, n, pc);
-  column_flags(st, static_cast<const float2*>(j.prop), w.col_flags, n);
-  HIP_OK(hipGetLastError());
-  std::vector<int> col_nz(n);
-  HIP_OK(hipMemcpyAsync(col_nz.data(), w.col_flags, n * sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_OK(hipStreamSynchronize(st));
   JobDev jd;
-  column_band(col_nz.data(), n, pc, jd.band_lo, jd.band_len);
-  // the columns pass runs the band's groups, rounded up to a multiple of 8 so that
-  // its XCD order applies (the added groups hold zeros and are transformed as such)
-  int band_groups = jd.band_len / pc;
-  if (band_groups < n / pc) {
-    band_groups = std::min(n / pc, (band_groups + 7) / 8 * 8);
-    jd.band_len = band_groups * pc;
-  }
+  job_band(st, jd, static_cast<const float2*>(j.prop), w.col_flags, n, pc);
+  const int band_groups = std::min(jd.band_ng, n / pc);  // columns-pass groups per probe
   jd.t = static_cast<const float2*>(j.t);
   jd.prop = w.prop;
   jd.prop_cols = pc;
@@ -994,12 +1008,9 @@ static void run_stem_impl(const totem_stem_job& j, int batch, int flags, void* w
   HIP_OK(hipMemcpyAsync(w.job, &jd, sizeof(jd), hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(w.mid_of_slice, mos.data(), S * sizeof(int), hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(w.layer_hit, hit.data(), hit.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  // detector partials of the groups outside the band and the exit wave's columns
-  // there are never written by the passes: they start (and stay) zero
-  if (jd.band_len < n) {
-    fill(st, w.partial, (long)batch * (L + 1) * A * Gslot, 0.0);
-    if (j.exit_wave) fill(st, static_cast<float2*>(j.exit_wave), total * n * n, 0.f, 0.f);
-  }
+  // the exit wave's columns outside the band are never written by the passes (the
+  // detector reduction reads the band groups' partials only)
+  if (jd.band_len < n && j.exit_wave) fill(st, static_cast<float2*>(j.exit_wave), total * n * n, 0.f, 0.f);
   HIP_OK(hipStreamSynchronize(st));  // host staging buffers above are locals
   // every probe's work item, uploaded once: chunk c runs items [c*batch, ...) in
   // wave slots 0..batch-1, which consecutive chunks reuse in stream order

@@ -32,6 +32,10 @@ struct JobDev {
   // only).  band_lo and band_len are multiples of the columns pass's group width.
   int band_lo = 0;
   int band_len = 1 << 30;
+  // the same band in column groups of the columns pass: groups (g - band_g0) mod G <
+  // band_ng; the columns pass runs and the detector reduction sums only these
+  int band_g0 = 0;
+  int band_ng = 1 << 30;
 };
 
 // One probe position of one job (a "work item") and the device memory it owns

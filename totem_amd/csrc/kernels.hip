@@ -518,10 +518,11 @@ __device__ __forceinline__ ColUnit col_unit(const ItemDev* __restrict__ items, c
   unit_map<N, K>(blockIdx.x, u, groups, item, gi);
   const ItemDev it = items[item];
   r.jb = &jobs[it.job];
-  r.cg = gi + r.jb->band_lo / Cfg::C;
+  r.cg = gi + r.jb->band_g0;
   if (r.cg >= G) r.cg -= G;
   r.slice = step - it.start_step;
-  r.active = r.slice >= 0 && r.slice < r.jb->S;
+  // a launch runs the largest band of its jobs; groups beyond a job's own band are idle
+  r.active = r.slice >= 0 && r.slice < r.jb->S && gi < r.jb->band_ng;
   r.wr = buf_rsrc(it.wave, (unsigned)N * N * sizeof(float2));
   r.last = r.slice == r.jb->S - 1;
   r.mid = (r.active && r.jb->mid_of_slice) ? r.jb->mid_of_slice[r.slice] : -1;
@@ -645,6 +646,9 @@ __global__ void k_reduce_det(const ItemDev* __restrict__ items, const int* __res
     const double* p = it.part + ((size_t)layer * jb.A + a) * G;
     double comp = 0.0;
     for (int g = 0; g < G; ++g) {
+      int d = g - jb.band_g0;  // only the job's band groups are written (the rest is zero)
+      if (d < 0) d += G;
+      if (d >= jb.band_ng) continue;
       const double y = p[g] - comp;
       const double tt = s + y;
       comp = (tt - s) - y;

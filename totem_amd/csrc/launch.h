@@ -33,6 +33,15 @@ int pass_cols_width(int n, int v);
 void prop_column_groups(hipStream_t st, const float2* prop, float2* out, int n, int C);
 // flags[c] = 1 if column c of prop [n][n] (row-major) holds a nonzero entry, else 0.
 void column_flags(hipStream_t st, const float2* prop, int* flags, int n);
+// The shortest wrapped run of C-column groups covering every flagged column (len = 0
+// without a flag, the whole grid without an empty group).
+void column_band(const int* flags, int n, int C, int& lo, int& len);
+// The band a run processes: column_band widened at its end to a multiple of 8
+// groups (at most the grid), in columns and in groups, into the job entry.
+void set_job_band(JobDev& jd, const int* flags, int n, int C);
+// The job's band of prop (column flags on `st`, which is synchronised) into jd;
+// flags: n ints of device scratch.
+void job_band(hipStream_t st, JobDev& jd, const float2* prop, int* flags, int n, int C);
 void pass_rows(int n, int variant, hipStream_t st, const ItemDev* items, int n_items, const JobDev* jobs, int step);
 // The columns pass runs `groups` column groups per item, from the first group of each
 // job's band on (all n / pass_cols_width(n, variant) of them for the default band).

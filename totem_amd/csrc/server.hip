@@ -62,6 +62,7 @@ struct Job {
   long total = 0, next = 0, done = 0;
   long long started = 0;     // next job id when admission first considered the job (0: not yet)
   std::vector<int> mos;
+  int band_ng = 0;           // column groups of the job's band (the columns pass runs these)
   hipEvent_t ev = nullptr;   // recorded after the job's last reduction
   bool closed = false;
 };
@@ -99,7 +100,7 @@ struct totem_server {
   int max_aper = 0, max_mid = 0;
   size_t page_bytes = 0;
   int n_pages = 0;
-  totem::DevBuf arena, jobs_dev, meta_dev;
+  totem::DevBuf arena, jobs_dev, meta_dev, col_flags;
   std::vector<int> free_pages;
   std::vector<hipEvent_t> page_ev;    // recorded on the releasing stream when a page returns to the pool
   std::vector<bool> page_ev_set;
@@ -270,7 +271,11 @@ void step_group(Server& s, Group& g) {
   const ItemDev* items = g.items_dev.as<ItemDev>();
   const JobDev* jobs = s.jobs_dev.as<JobDev>();
   pass_rows(g.n, g.plan, g.st, items, cnt, jobs, g.step);
-  pass_cols(g.n, g.plan, g.st, items, cnt, jobs, g.step, g.n / pass_cols_width(g.n, g.plan));
+  // the columns pass runs the largest band of the jobs in flight (each job skips the
+  // groups beyond its own)
+  int groups = 0;
+  for (const Cohort& c : g.cohorts) groups = std::max(groups, s.jobs.at(c.job).band_ng);
+  if (groups > 0) pass_cols(g.n, g.plan, g.st, items, cnt, jobs, g.step, groups);
   bool removed = false;
   for (Cohort& c : g.cohorts) {
     Job& J = s.jobs.at(c.job);
@@ -329,13 +334,22 @@ void submit_impl(Server& s, const totem_stem_job& j, long long* id) {
   jd.n_out = (int)J.total;
   jd.wrap = j.wrap;
   jd.prop_cols = 0;  // the caller's row-major propagator
+  // the job's band, as a run alone finds it (the same groups, so the same images); the
+  // null stream does not wait for the server's (non-blocking) streams
+  Group& g = group_of(s, j.n);
+  const int C = pass_cols_width(j.n, g.plan);
+  job_band(nullptr, jd, jd.prop, s.col_flags.as<int>(), j.n, C);
+  J.band_ng = std::min(jd.band_ng, j.n / C);
+  if (jd.band_len < j.n && j.exit_wave) {  // exit-wave columns outside the band stay zero
+    fill(nullptr, jd.exit, J.total * j.n * j.n, 0.f, 0.f);
+    HIP_OK(hipStreamSynchronize(nullptr));
+  }
   // the entry is free (its previous job completed), so a blocking copy cannot race a kernel
   HIP_OK(hipMemcpy(meta, J.mos.data(), J.mos.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(meta + kMaxSlices, hit.data(), hit.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(s.jobs_dev.as<JobDev>() + J.table, &jd, sizeof(jd), hipMemcpyHostToDevice));
   HIP_OK(hipEventCreateWithFlags(&J.ev, hipEventDisableTiming));
   s.free_tables.pop_back();
-  group_of(s, j.n);
   s.order.push_back(J.id);
   *id = J.id;
   s.jobs.emplace(J.id, std::move(J));
@@ -364,6 +378,7 @@ int totem_server_create(size_t arena_bytes, int max_aper, int max_mid, totem_ser
     s->arena.alloc((size_t)s->n_pages * s->page_bytes);
     s->jobs_dev.alloc(totem::srv::kMaxJobs * sizeof(totem::JobDev));
     s->meta_dev.alloc((size_t)totem::srv::kMaxJobs * s->meta_ints() * sizeof(int));
+    s->col_flags.alloc(totem::srv::kMaxSide * sizeof(int));
     for (int p = s->n_pages - 1; p >= 0; --p) s->free_pages.push_back(p);
     s->page_ev.resize(s->n_pages);
     s->page_ev_set.assign(s->n_pages, false);
