@@ -494,7 +494,7 @@ def main():
         kname, kavg, kbytes = "k_pass_cols<%d>" % N, cols_avg, bytes_cols
     achieved = kbytes / (kavg * 1e-3) / 1e9 if kavg > 0 else None
     traffic = measured_traffic(kname, N, args.workload)
-    # the other pass of the slice, and both together (per probe.slice: 32 N^2 bytes + shared slices)
+    # the other pass of the slice, and both together (per probe.slice: 32 f N^2 bytes + shared slices)
     oname, oavg, obytes = (("k_pass_cols<%d>" % N, cols_avg, bytes_cols) if kname.startswith("k_pass_rows")
                            else ("k_pass_rows<%d>" % N, rows_avg, bytes_rows))
     oach = obytes / (oavg * 1e-3) / 1e9 if oavg > 0 else None

@@ -153,7 +153,9 @@ typedef struct totem_stem_job {
  * reference, which fills free memory, cuda_source_lobato.cu:1249-1260). */
 int totem_stem_preferred_batch(int n);
 /* Measures every plan of n x n grids with probes per launch around the cache-fit
- * rule on the current device (a synthetic job through `slices` <= 64 slices) and
+ * rule on the current device (a synthetic job through `slices` <= 64 slices, its
+ * propagator zero beyond the 2/3 band limit as ToTEM's is, so that the runs move the
+ * band a real run moves) and
  * makes the fastest plan and batch the device's for n from then on; returns the
  * batch and its probe.slice/s (may be NULL); totem_stem_plan reports the plan.
  * Allocates and synchronises; no counterpart in the reference. */

@@ -1156,8 +1156,20 @@ static int autotune_impl(int n, int slices, double* best_rate) {
   DevBuf aper(nn * sizeof(double)), det((size_t)2 * bmax * sizeof(double));
   fill(st, t.as<float2>(), (long)(slices * nn), 1.f, 0.f);
   fill(st, probe.as<float2>(), (long)nn, 1.f, 0.f);
-  fill(st, prop.as<float2>(), (long)nn, 1.f, 0.f);
   fill(st, aper.as<double>(), (long)nn, 0.0);
+  {
+    // a propagator with the support of ToTEM's (the 2/3 band limit, |k| <= n/3 in
+    // FFT order), so that the tuned runs move and transform the band a real run does
+    std::vector<float2> h(nn);
+    for (int r = 0; r < n; ++r)
+      for (int c = 0; c < n; ++c) {
+        const long kr = r < n / 2 ? r : r - n, kc = c < n / 2 ? c : c - n;
+        const bool in = 9 * (kr * kr + kc * kc) <= (long)n * n;
+        h[(size_t)r * n + c] = make_float2(in ? 1.f : 0.f, 0.f);
+      }
+    HIP_OK(hipMemcpyAsync(prop.p, h.data(), nn * sizeof(float2), hipMemcpyHostToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
+  }
   totem_stem_job j;
   std::memset(&j, 0, sizeof(j));
   j.n = n;

@@ -9,7 +9,10 @@
 //  * k_fft_rows / k_fft_cols -- plain batched 2-D FFT passes (potential IFFT, tests)
 //
 // A full 2-D forward+inverse FFT per slice therefore costs two read+write
-// passes over the probe wave in HBM (32 N^2 bytes per probe.slice).
+// passes over the probe wave in HBM: 32 f N^2 bytes per probe.slice, where f is the
+// fraction of reciprocal-space columns in which the propagator has a nonzero entry
+// (the wave is zero elsewhere after every propagator multiply, so neither pass
+// reads or writes those columns; 2/3 with the 2/3 band limit).
 #include "fft_block.h"
 #include "host_common.h"
 #include "kernels.h"
