@@ -17,6 +17,7 @@ on a bounded sample of the same workload.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -75,6 +76,19 @@ def measured_traffic(kname, n, workload):
                 f = line[len(kname):].split()
                 return (float(f[1]) + float(f[3])) * 2**20
     return None
+
+
+def measured_valu_issue(kname, n, workload):
+    """Fraction of the kernel's duration its VALU instructions take to issue (SQ_INSTS_VALU x 4
+    cycles / 1024 SIMDs / 2.4 GHz over the kernel-trace duration), from the committed SQ counter
+    summary, or None when no summary covers this workload."""
+    path = os.path.join(ROOT, "profiles", "stem_passes_%d_%s_sq_counters.txt" % (n, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        text = fh.read()
+    m = re.search(re.escape(kname) + r":[^\n]*\n\s*VALU issue time [\d.]+ us .*?= (\d+)% of", text)
+    return int(m.group(1)) / 100.0 if m else None
 
 
 def run_hrtem_batch(n_images, cpu_images, seed, n=320):
@@ -529,10 +543,12 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": kavg,
                      "band_columns": band, "band_fraction": f,
+                     "valu_issue_frac": measured_valu_issue(kname, N, args.workload),
                      "rows_avg_ms": rows_avg, "cols_avg_ms": cols_avg},
         "roofline_other_pass": {"kernel": oname, "achieved": oach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": (oach / HBM_PEAK_GBS) if oach else None,
-                                "traffic": measured_traffic(oname, N, args.workload), "avg_launch_ms": oavg},
+                                "traffic": measured_traffic(oname, N, args.workload), "avg_launch_ms": oavg,
+                                "valu_issue_frac": measured_valu_issue(oname, N, args.workload)},
         "roofline_both_passes": {"achieved": both, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": (both / HBM_PEAK_GBS) if both else None},
     }

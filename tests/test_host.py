@@ -242,6 +242,10 @@ def test_bench_reads_committed_traffic_profile():
         assert tr is not None, k
         assert 0.95 * alg < tr < 1.15 * alg, (k, tr / alg)
     assert bench.measured_traffic("k_pass_rows<512>", 512, "c2") is None
+    # and the VALU issue fraction of each pass from the committed SQ counter summary
+    for k in algs:
+        v = bench.measured_valu_issue(k, 1024, "c2")
+        assert v is not None and 0.3 < v < 1.0, (k, v)
 
 
 # --------------------------------------------------------------------------- potential planner
