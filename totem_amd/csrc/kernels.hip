@@ -919,16 +919,26 @@ __global__ void k_prop_cols(const float2* __restrict__ prop, float2* __restrict_
   out[((long)g * n + r) * C + c] = prop[i];
 }
 
-// One thread per column; consecutive threads read consecutive entries of a row.
+// flags[c] = 0 for every column, then one thread per (column, chunk of n / kFlagChunks
+// rows) sets it to 1 on a nonzero entry (every writer stores the same value);
+// consecutive threads read consecutive entries of a row.
+constexpr int kFlagChunks = 32;
+__global__ void k_zero_i32(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
 __global__ void k_column_flags(const float2* __restrict__ prop, int* __restrict__ flags, int n) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  int nz = 0;
-  for (int r = 0; r < n && !nz; ++r) {
+  const int rows = (n + kFlagChunks - 1) / kFlagChunks;
+  const int r0 = blockIdx.y * rows, r1 = min(n, r0 + rows);
+  for (int r = r0; r < r1; ++r) {
     const float2 h = prop[(size_t)r * n + c];
-    nz = h.x != 0.f || h.y != 0.f;
+    if (h.x != 0.f || h.y != 0.f) {
+      flags[c] = 1;
+      return;
+    }
   }
-  flags[c] = nz;
 }
 
 __global__ void k_fill_c64(float2* __restrict__ p, long n, float re, float im) {
@@ -1164,7 +1174,9 @@ void prop_column_groups(hipStream_t st, const float2* prop, float2* out, int n, 
 }
 
 void column_flags(hipStream_t st, const float2* prop, int* flags, int n) {
-  hipLaunchKernelGGL(k_column_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, prop, flags, n);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_zero_i32, dim3(nb), dim3(256), 0, st, flags, n);
+  hipLaunchKernelGGL(k_column_flags, dim3(nb, kFlagChunks), dim3(256), 0, st, prop, flags, n);
 }
 
 void fill(hipStream_t st, float2* p, long n, float re, float im) {
