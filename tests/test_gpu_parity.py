@@ -308,6 +308,24 @@ def test_stem_with_shifted_propagator_band_matches_oracle(engine, n, shift):
         assert rel.max() <= 1e-4
 
 
+def test_band_limited_run_equals_full_grid_run(engine):
+    """The passes skip the columns where the propagator is zero: the same job with 1e-30 in
+    every zero entry of the propagator (so no column is zero and the whole grid is transformed)
+    gives the same detector images to 1e-6 -- a check of the skipping that does not go through
+    the oracle's tolerance."""
+    import dataclasses
+    print("case band vs full grid", flush=True)
+    cfg, inp = _stem_case("n", M=1, mid=(2, 4), n=512, np_side=1024, cells=2, scan=(2, 2), step=9,
+                          begin=(11, 23))
+    C = engine.plan_columns(512, engine.plan(512)[0])
+    full = np.where(inp.psf == 0, 1e-30, inp.psf)
+    assert engine.stem_band(inp.psf, C)[1] < 512 and engine.stem_band(full, C)[1] == 512
+    r_band, m_band, _, _ = stem.run(inp)
+    r_full, m_full, _, _ = stem.run(dataclasses.replace(inp, psf=full))
+    np.testing.assert_allclose(r_band, r_full, rtol=1e-6, atol=0)
+    np.testing.assert_allclose(m_band, m_full, rtol=1e-6, atol=0)
+
+
 def test_zero_propagator_gives_zero_images(engine):
     """A propagator without a nonzero entry: an empty band, every detector image exactly zero (as
     the oracle's)."""
