@@ -544,6 +544,11 @@ def main():
                      "algorithmic_bytes_per_launch": kbytes, "avg_launch_ms": kavg,
                      "band_columns": band, "band_fraction": f,
                      "valu_issue_frac": measured_valu_issue(kname, N, args.workload),
+                     # the full-grid transform's bytes for the same launch over its measured time
+                     # (the rate at which the pass advances probe.slices, in those bytes)
+                     "full_grid_equivalent_frac":
+                         ((batch * 16.0 * n2 + 8.0 * (NP * NP if kname.startswith("k_pass_rows") else n2))
+                          / (kavg * 1e-3) / 1e9 / HBM_PEAK_GBS) if kavg > 0 else None,
                      "rows_avg_ms": rows_avg, "cols_avg_ms": cols_avg},
         "roofline_other_pass": {"kernel": oname, "achieved": oach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": (oach / HBM_PEAK_GBS) if oach else None,
